@@ -1,0 +1,159 @@
+"""ctypes binding of the C ABI in include/tm_abi.h (libtm_hip.so).
+
+The library is loaded from everest_amd/lib/; if it is missing the import of
+this module succeeds but every call raises, so no code path silently falls
+back to a CPU implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libtm_hip.so")
+MAX_E = 8
+
+TM_OK = 0
+STATUS = {0: "TM_OK", 1: "TM_ERR_ARG", 2: "TM_ERR_MOTIF", 3: "TM_ERR_HIP", 4: "TM_ERR_NOMEM",
+          5: "TM_ERR_INTERNAL", 6: "TM_ERR_NODEVICE"}
+
+
+class TMError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+        self.status = status
+
+
+class HostGraph(ctypes.Structure):
+    _fields_ = [("num_edges", ctypes.c_int32), ("num_vertices", ctypes.c_int32),
+                ("Eg", ctypes.c_void_p), ("inEdgesV", ctypes.c_void_p), ("outEdgesV", ctypes.c_void_p),
+                ("inEdgesR", ctypes.c_void_p), ("outEdgesR", ctypes.c_void_p), ("Euid", ctypes.c_void_p)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_roots", ctypes.c_int32),
+                ("scan_max", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libtm_hip.so is not built (expected at %s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        P, I32, I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        L.tm_abi_version.restype = I32
+        L.tm_last_error.restype = ctypes.c_char_p
+        L.tm_device_count.restype = I32
+        L.tm_graph_upload.argtypes = [I32, ctypes.POINTER(HostGraph), ctypes.POINTER(P)]
+        L.tm_graph_free.argtypes = [P]
+        L.tm_graph_info.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I32), ctypes.POINTER(I32),
+                                    ctypes.POINTER(I32)]
+        L.tm_count.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P]
+        L.tm_enum.argtypes = [P, I32, P, P, I32, I32, I32, I64, ctypes.POINTER(Tuning), P, P,
+                              ctypes.POINTER(I64)]
+        L.tm_batch_count.argtypes = [P, I32, P, P, P, P, ctypes.POINTER(Tuning), P]
+        for name in ("tm_graph_upload", "tm_graph_free", "tm_graph_info", "tm_count", "tm_enum",
+                     "tm_batch_count"):
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != TM_OK:
+        raise TMError(status, lib().tm_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def tuning(waves_per_cu=0, chunk_roots=0, scan_max=0):
+    t = Tuning()
+    t.waves_per_cu, t.chunk_roots, t.scan_max = waves_per_cu, chunk_roots, scan_max
+    return t
+
+
+def motif_arrays(edges):
+    """(src, dst) int32 arrays of a renamed motif given as (a, b[, k]) tuples."""
+    src = np.array([e[0] for e in edges], dtype=np.int32)
+    dst = np.array([e[1] for e in edges], dtype=np.int32)
+    return src, dst
+
+
+class DeviceGraph:
+    """A graph resident in HBM (tm_graph handle)."""
+
+    def __init__(self, Eg, inV, outV, inR, outR, Euid, device=0):
+        L = lib()
+        self._keep = [np.ascontiguousarray(a, dtype=np.int32) for a in (Eg, inV, outV, inR, outR, Euid)]
+        Eg, inV, outV, inR, outR, Euid = self._keep
+        hg = HostGraph(int(Eg.size // 3), int(inR.size - 1), _p(Eg), _p(inV), _p(outV), _p(inR), _p(outR),
+                       _p(Euid))
+        h = ctypes.c_void_p()
+        check(L.tm_graph_upload(int(device), ctypes.byref(hg), ctypes.byref(h)))
+        self.handle = h
+        self.num_edges = int(Eg.size // 3)
+        self._keep = None
+
+    @classmethod
+    def from_dict(cls, g, device=0):
+        return cls(g["Eg"], g["inV"], g["outV"], g["inR"], g["outR"], g["Euid"], device)
+
+    def info(self):
+        b, e, v, d = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().tm_graph_info(self.handle, ctypes.byref(b), ctypes.byref(e), ctypes.byref(v),
+                                  ctypes.byref(d)))
+        return dict(device_bytes=b.value, num_edges=e.value, num_vertices=v.value, device=d.value)
+
+    def count(self, edges, delta, beg=0, end=-1, tune=None):
+        src, dst = motif_arrays(edges)
+        out = np.zeros(1, dtype=np.uint64)
+        t = tune if tune is not None else Tuning()
+        check(lib().tm_count(self.handle, len(edges), _p(src), _p(dst), int(delta), int(beg), int(end),
+                             ctypes.byref(t), _p(out)))
+        return int(out[0])
+
+    def enum(self, edges, delta, cap, beg=0, end=-1, tune=None):
+        src, dst = motif_arrays(edges)
+        M = len(edges)
+        rows = np.zeros((max(int(cap), 1), M), dtype=np.int32)
+        cnt = np.zeros(1, dtype=np.uint64)
+        nw = ctypes.c_int64()
+        t = tune if tune is not None else Tuning()
+        check(lib().tm_enum(self.handle, M, _p(src), _p(dst), int(delta), int(beg), int(end), int(cap),
+                            ctypes.byref(t), _p(rows), _p(cnt), ctypes.byref(nw)))
+        return int(cnt[0]), rows[:nw.value].copy()
+
+    def batch_count(self, queries, tune=None):
+        """queries: list of (edges, delta)."""
+        nq = len(queries)
+        ne = np.zeros(nq, dtype=np.int32)
+        src = np.zeros((nq, MAX_E), dtype=np.int32)
+        dst = np.zeros((nq, MAX_E), dtype=np.int32)
+        dl = np.zeros(nq, dtype=np.int32)
+        for q, (edges, delta) in enumerate(queries):
+            ne[q] = len(edges)
+            s, d = motif_arrays(edges)
+            src[q, :len(edges)] = s
+            dst[q, :len(edges)] = d
+            dl[q] = delta
+        out = np.zeros(nq, dtype=np.uint64)
+        t = tune if tune is not None else Tuning()
+        check(lib().tm_batch_count(self.handle, nq, _p(ne), _p(src), _p(dst), _p(dl), ctypes.byref(t), _p(out)))
+        return [int(x) for x in out]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tm_graph_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

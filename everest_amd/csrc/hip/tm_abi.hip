@@ -1,0 +1,523 @@
+// tm_abi.hip -- host driver of the gfx950 temporal-motif kernels, behind the
+// C ABI declared in include/tm_abi.h.
+//
+// Graph residency: a graph is uploaded once and stays in HBM with the
+// reference arrays (Eg, in/out CSR, Euid) plus indexes built on the device at
+// upload: packed times T, neighbour arrays aligned with the CSR position
+// lists, a per-node (neighbour, position)-sorted copy of each list (the pair
+// index), and a time-bucket index for the window end.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tm_abi.h"
+#include "tm_kernel.cuh"
+#include "tm_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+tm_status fail(tm_status s, const std::string &msg) {
+  g_last_error = msg;
+  return s;
+}
+
+#define HIPTRY(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? TM_ERR_NOMEM : TM_ERR_HIP,               \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+__global__ void aux_kernel(const tm_edge *__restrict__ Eg, const int *__restrict__ outPos,
+                           const int *__restrict__ inPos, int E, int *__restrict__ T,
+                           int *__restrict__ outNbr, int *__restrict__ inNbr,
+                           unsigned long long *__restrict__ okey, unsigned long long *__restrict__ ikey) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= E) return;
+  T[i] = Eg[i].t;
+  int po = outPos[i];
+  tm_edge eo = Eg[po];
+  outNbr[i] = eo.v;
+  okey[i] = ((unsigned long long)(unsigned)eo.u << 32) | (unsigned)eo.v;
+  int pi = inPos[i];
+  tm_edge ei = Eg[pi];
+  inNbr[i] = ei.u;
+  ikey[i] = ((unsigned long long)(unsigned)ei.v << 32) | (unsigned)ei.u;
+}
+
+__global__ void low_word_kernel(const unsigned long long *__restrict__ k, int *__restrict__ out, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int)(unsigned)(k[i] & 0xffffffffull);
+}
+
+__global__ void bucket_kernel(const int *__restrict__ T, int E, int tmin, int bshift, long long nb,
+                              int *__restrict__ bucket) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nb) return;
+  long long key = (long long)tmin + (k << bshift);
+  int lo = 0, hi = E;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if ((long long)T[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  bucket[k] = lo;
+}
+
+__global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) { s->next_chunk = 0; s->error = 0; }
+  if (i < n) { counts[i] = 0; stops[i] = 0; }
+}
+
+int ceil_log2(long long x) {
+  int r = 0;
+  while ((1ll << r) < x) r++;
+  return r;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- graph
+
+struct tm_graph {
+  int device = 0;
+  int E = 0, V = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+  tm_graph_view view{};
+  tmh::Scratch scratch;
+  std::mutex mtx;  // one launch at a time per graph handle
+};
+
+namespace tmh {
+
+tm_status set_error(tm_status s, const std::string &msg) { return fail(s, msg); }
+
+tm_status dev_alloc(tm_graph *g, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 4;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? TM_ERR_NOMEM : TM_ERR_HIP,
+                std::string("hipMalloc: ") + hipGetErrorString(e));
+  g->allocs.push_back(*p);
+  g->bytes += (int64_t)bytes;
+  return TM_OK;
+}
+
+// Build the device descriptor of a motif given as renamed (src, dst) pairs.
+tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_dev *md) {
+  if (M < 2 || M > TM_MAXE) return fail(TM_ERR_MOTIF, "motif must have 2..8 edges");
+  memset(md, 0, sizeof(*md));
+  md->M = M;
+  if (src[0] != 0 || dst[0] != 1)
+    return fail(TM_ERR_MOTIF, "motif nodes must be renamed by first appearance (edge 0 = 0->1)");
+  int n = 2;
+  for (int j = 1; j < M; j++) {
+    int a = src[j], b = dst[j];
+    if (a < 0 || b < 0) return fail(TM_ERR_MOTIF, "negative motif node id");
+    if (a == b) return fail(TM_ERR_MOTIF, "motif self-loop");
+    md->a[j] = a;
+    md->b[j] = b;
+    md->mapped[j] = n;
+    if (a < n && b < n) {
+      md->kind[j] = TM_K_CLOSE;
+    } else if (a < n && b == n) {
+      md->kind[j] = TM_K_NEW_DST;
+      n++;
+    } else if (b < n && a == n) {
+      md->kind[j] = TM_K_NEW_SRC;
+      n++;
+    } else {
+      return fail(TM_ERR_MOTIF, "motif is not prefix-connected with first-appearance node ids");
+    }
+    if (n > TM_MAXN) return fail(TM_ERR_MOTIF, "motif has more than 8 nodes");
+  }
+  md->n = n;
+  return TM_OK;
+}
+
+template <int MAXN, bool ENUM>
+static tm_status occupancy(int lds, int *blocks) {
+  HIPTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, tmk::mine_kernel<TM_MAXE, MAXN, ENUM>, 64, lds));
+  return TM_OK;
+}
+
+template <int MAXN, bool ENUM>
+static tm_status launch_t(tm_graph *g, const tm_kernel_args &a, int grid, int lds) {
+  hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM>), dim3(grid), dim3(64), lds, g->stream, a);
+  HIPTRY(hipGetLastError());
+  return TM_OK;
+}
+
+// Plan and launch the mining kernel for queries already staged in scratch.
+tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const tm_tuning *tune,
+                 long long total_chunks, int chunk_roots) {
+  Scratch &s = g->scratch;
+  const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
+  const int frames = 64 * lvls;
+  const int nmap = maxN <= 4 ? 4 : 8;
+  const int per_frame = 5 + nmap + (enumerate ? TM_MAXE : 0);
+  const int lds = frames * per_frame * (int)sizeof(int);
+  int blocks = 0;
+  tm_status st;
+  if (nmap == 4) st = enumerate ? occupancy<4, true>(lds, &blocks) : occupancy<4, false>(lds, &blocks);
+  else st = enumerate ? occupancy<8, true>(lds, &blocks) : occupancy<8, false>(lds, &blocks);
+  if (st != TM_OK) return st;
+  int wpc = (tune && tune->waves_per_cu > 0) ? tune->waves_per_cu : 32;
+  if (blocks > 0) wpc = std::min(wpc, blocks);
+  if (wpc < 1) wpc = 1;
+  int grid = g->num_cus * wpc;
+  if (total_chunks < grid) grid = (int)std::max<long long>(1, total_chunks);
+
+  tm_kernel_args a;
+  memset(&a, 0, sizeof(a));
+  a.g = g->view;
+  a.queries = s.d_queries;
+  a.nq = nq;
+  a.chunk_roots = chunk_roots;
+  a.chunk_prefix = s.d_prefix;
+  a.total_chunks = total_chunks;
+  a.frames = frames;
+  a.scan_max = (tune && tune->scan_max > 0) ? tune->scan_max : 16;
+  a.sched = s.d_sched;
+
+  hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, g->stream, s.d_sched,
+                     s.d_counts, s.d_stops, nq);
+  HIPTRY(hipGetLastError());
+  if (nmap == 4) st = enumerate ? launch_t<4, true>(g, a, grid, lds) : launch_t<4, false>(g, a, grid, lds);
+  else st = enumerate ? launch_t<8, true>(g, a, grid, lds) : launch_t<8, false>(g, a, grid, lds);
+  return st;
+}
+
+// Reserve query staging space for nq queries (grows, never shrinks).
+tm_status reserve(tm_graph *g, int nq) {
+  Scratch &s = g->scratch;
+  if (nq <= s.cap_q) return TM_OK;
+  int cap = std::max(nq, 256);
+  void *p;
+  tm_status st;
+  if ((st = dev_alloc(g, &p, sizeof(tm_query_dev) * cap)) != TM_OK) return st;
+  s.d_queries = (tm_query_dev *)p;
+  if ((st = dev_alloc(g, &p, sizeof(long long) * (cap + 1))) != TM_OK) return st;
+  s.d_prefix = (long long *)p;
+  if ((st = dev_alloc(g, &p, sizeof(unsigned long long) * cap)) != TM_OK) return st;
+  s.d_counts = (unsigned long long *)p;
+  if ((st = dev_alloc(g, &p, sizeof(int) * cap)) != TM_OK) return st;
+  s.d_stops = (int *)p;
+  if (!s.d_sched) {
+    if ((st = dev_alloc(g, &p, sizeof(tm_sched))) != TM_OK) return st;
+    s.d_sched = (tm_sched *)p;
+  }
+  s.cap_q = cap;
+  s.h_queries.resize(cap);
+  s.h_prefix.resize(cap + 1);
+  return TM_OK;
+}
+
+}  // namespace tmh
+
+using namespace tmh;
+
+extern "C" {
+
+int32_t tm_abi_version(void) { return TM_ABI_VERSION; }
+
+const char *tm_last_error(void) { return g_last_error.c_str(); }
+
+int32_t tm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **out) {
+  if (!hg || !out) return fail(TM_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int E = hg->num_edges, V = hg->num_vertices;
+  if (E < 1 || V < 1) return fail(TM_ERR_ARG, "graph needs at least one edge and vertex");
+  if (!hg->Eg || !hg->inEdgesV || !hg->outEdgesV || !hg->inEdgesR || !hg->outEdgesR || !hg->Euid)
+    return fail(TM_ERR_ARG, "null graph array");
+  // Host-side validation: every index the kernels follow must be in range.
+  const tm_edge *Eg = (const tm_edge *)hg->Eg;
+  for (int p = 0; p < E; p++) {
+    if (Eg[p].u < 0 || Eg[p].u >= V || Eg[p].v < 0 || Eg[p].v >= V)
+      return fail(TM_ERR_ARG, "edge endpoint out of [0, V)");
+    if (p && Eg[p].t < Eg[p - 1].t) return fail(TM_ERR_ARG, "edges are not sorted by time");
+  }
+  for (int k = 0; k < 2; k++) {
+    const int32_t *R = k ? hg->inEdgesR : hg->outEdgesR;
+    const int32_t *P = k ? hg->inEdgesV : hg->outEdgesV;
+    if (R[0] != 0 || R[V] != E) return fail(TM_ERR_ARG, "CSR offsets must span [0, E]");
+    for (int x = 0; x < V; x++) {
+      if (R[x + 1] < R[x]) return fail(TM_ERR_ARG, "CSR offsets not monotone");
+      for (int s = R[x]; s < R[x + 1]; s++) {
+        int p = P[s];
+        if (p < 0 || p >= E) return fail(TM_ERR_ARG, "CSR position out of range");
+        if ((k ? Eg[p].v : Eg[p].u) != x) return fail(TM_ERR_ARG, "CSR position on the wrong vertex");
+        if (s > R[x] && P[s - 1] >= p) return fail(TM_ERR_ARG, "CSR positions not ascending");
+      }
+    }
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(TM_ERR_NODEVICE, "no GPU visible");
+  if (device < 0 || device >= ndev) return fail(TM_ERR_ARG, "device out of range");
+  HIPTRY(hipSetDevice(device));
+
+  tm_graph *g = new tm_graph();
+  g->device = device;
+  g->E = E;
+  g->V = V;
+  hipDeviceProp_t prop;
+  hipError_t he = hipGetDeviceProperties(&prop, device);
+  if (he != hipSuccess) { delete g; return fail(TM_ERR_HIP, hipGetErrorString(he)); }
+  g->num_cus = prop.multiProcessorCount;
+  he = hipStreamCreate(&g->stream);
+  if (he != hipSuccess) { delete g; return fail(TM_ERR_HIP, hipGetErrorString(he)); }
+
+  auto cleanup = [&](tm_status s) {
+    tm_graph_free(g);
+    return s;
+  };
+  const size_t e4 = sizeof(int) * (size_t)E, v4 = sizeof(int) * (size_t)(V + 1);
+  void *p;
+  tm_graph_view &vw = g->view;
+  vw.E = E;
+  vw.V = V;
+  tm_status st;
+#define ALLOC(field, bytes)                                            \
+  if ((st = dev_alloc(g, &p, (bytes))) != TM_OK) return cleanup(st); \
+  field = (decltype(field))p;
+  ALLOC(vw.Eg, 3 * e4);
+  ALLOC(vw.T, e4);
+  ALLOC(vw.outR, v4);
+  ALLOC(vw.inR, v4);
+  ALLOC(vw.outPos, e4);
+  ALLOC(vw.inPos, e4);
+  ALLOC(vw.outNbr, e4);
+  ALLOC(vw.inNbr, e4);
+  ALLOC(vw.outPairNbr, e4);
+  ALLOC(vw.outPairPos, e4);
+  ALLOC(vw.inPairNbr, e4);
+  ALLOC(vw.inPairPos, e4);
+  ALLOC(vw.Euid, e4);
+  const int tmin = Eg[0].t, tmax = Eg[E - 1].t;
+  const long long span = (long long)tmax - tmin + 1;
+  int bshift = ceil_log2(std::max(1ll, (span * 16) / std::max(1, E)));
+  if (bshift > 30) bshift = 30;
+  const long long nb = (span + (1ll << bshift) - 1) >> bshift;
+  vw.tmin = tmin;
+  vw.bshift = bshift;
+  vw.nbuckets = nb;
+  ALLOC(vw.bucket, sizeof(int) * (size_t)(nb + 1));
+#undef ALLOC
+  hipStream_t s = g->stream;
+#define H2D(dst, src, bytes) \
+  if ((he = hipMemcpyAsync((void *)(dst), (src), (bytes), hipMemcpyHostToDevice, s)) != hipSuccess) \
+    return cleanup(fail(TM_ERR_HIP, hipGetErrorString(he)));
+  H2D(vw.Eg, hg->Eg, 3 * e4);
+  H2D(vw.outR, hg->outEdgesR, v4);
+  H2D(vw.inR, hg->inEdgesR, v4);
+  H2D(vw.outPos, hg->outEdgesV, e4);
+  H2D(vw.inPos, hg->inEdgesV, e4);
+  H2D(vw.Euid, hg->Euid, e4);
+#undef H2D
+  // temporaries for the pair index
+  unsigned long long *okey = nullptr, *ikey = nullptr, *skey = nullptr;
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+  auto free_tmp = [&]() {
+    if (okey) (void)hipFree(okey);
+    if (ikey) (void)hipFree(ikey);
+    if (skey) (void)hipFree(skey);
+    if (tmp) (void)hipFree(tmp);
+  };
+#define TMPCHK(expr) \
+  if ((he = (expr)) != hipSuccess) { free_tmp(); return cleanup(fail(he == hipErrorOutOfMemory ? TM_ERR_NOMEM : TM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(he))); }
+  TMPCHK(hipMalloc(&okey, sizeof(unsigned long long) * (size_t)E));
+  TMPCHK(hipMalloc(&ikey, sizeof(unsigned long long) * (size_t)E));
+  TMPCHK(hipMalloc(&skey, sizeof(unsigned long long) * (size_t)E));
+  hipLaunchKernelGGL(aux_kernel, dim3((E + 255) / 256), dim3(256), 0, s, vw.Eg, vw.outPos, vw.inPos, E,
+                     (int *)vw.T, (int *)vw.outNbr, (int *)vw.inNbr, okey, ikey);
+  TMPCHK(hipGetLastError());
+  const int end_bit = std::min(64, 32 + std::max(1, ceil_log2((long long)V + 1)));
+  TMPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, okey, skey, vw.outPos, (int *)vw.outPairPos,
+                                            E, 0, end_bit, s));
+  TMPCHK(hipMalloc(&tmp, tmp_bytes));
+  TMPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, okey, skey, vw.outPos, (int *)vw.outPairPos, E,
+                                            0, end_bit, s));
+  hipLaunchKernelGGL(low_word_kernel, dim3((E + 255) / 256), dim3(256), 0, s, skey, (int *)vw.outPairNbr, E);
+  TMPCHK(hipGetLastError());
+  TMPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, ikey, skey, vw.inPos, (int *)vw.inPairPos, E, 0,
+                                            end_bit, s));
+  hipLaunchKernelGGL(low_word_kernel, dim3((E + 255) / 256), dim3(256), 0, s, skey, (int *)vw.inPairNbr, E);
+  TMPCHK(hipGetLastError());
+  hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((nb + 1 + 255) / 256)), dim3(256), 0, s, vw.T, E, tmin,
+                     bshift, nb, (int *)vw.bucket);
+  TMPCHK(hipGetLastError());
+  TMPCHK(hipStreamSynchronize(s));
+#undef TMPCHK
+  free_tmp();
+  if ((st = reserve(g, 1)) != TM_OK) return cleanup(st);
+  *out = g;
+  return TM_OK;
+}
+
+tm_status tm_graph_free(tm_graph *g) {
+  if (!g) return TM_OK;
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (void *p : g->allocs) (void)hipFree(p);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return TM_OK;
+}
+
+tm_status tm_graph_info(const tm_graph *g, int64_t *device_bytes, int32_t *num_edges,
+                        int32_t *num_vertices, int32_t *device) {
+  if (!g) return fail(TM_ERR_ARG, "null graph");
+  if (device_bytes) *device_bytes = g->bytes;
+  if (num_edges) *num_edges = g->E;
+  if (num_vertices) *num_vertices = g->V;
+  if (device) *device = g->device;
+  return TM_OK;
+}
+
+static tm_status stage_queries(tm_graph *g, int nq, const int32_t *num_edges, const int32_t *msrc,
+                               const int32_t *mdst, int stride, const int32_t *deltas,
+                               const int32_t *begs, const int32_t *ends, int chunk_roots,
+                               long long *total_chunks, int *maxM, int *maxN) {
+  tm_status st;
+  if ((st = reserve(g, nq)) != TM_OK) return st;
+  Scratch &s = g->scratch;
+  long long acc = 0;
+  *maxM = 2;
+  *maxN = 2;
+  for (int q = 0; q < nq; q++) {
+    tm_query_dev &qd = s.h_queries[q];
+    memset(&qd, 0, sizeof(qd));
+    if ((st = build_motif(num_edges[q], msrc + (size_t)q * stride, mdst + (size_t)q * stride, &qd.m)) != TM_OK)
+      return st;
+    if (deltas[q] < 0) return fail(TM_ERR_ARG, "delta must be >= 0");
+    qd.delta = deltas[q];
+    int beg = begs ? begs[q] : 0;
+    int end = ends ? ends[q] : -1;
+    if (end < 0) end = g->E - qd.m.M + 1;
+    if (beg < 0) beg = 0;
+    if (end > g->E) end = g->E;
+    if (end < beg) end = beg;
+    qd.root_beg = beg;
+    qd.root_end = end;
+    qd.count = s.d_counts + q;
+    qd.stop = s.d_stops + q;
+    s.h_prefix[q] = acc;
+    acc += ((long long)(end - beg) + chunk_roots - 1) / chunk_roots;
+    *maxM = std::max(*maxM, qd.m.M);
+    *maxN = std::max(*maxN, qd.m.n);
+  }
+  s.h_prefix[nq] = acc;
+  *total_chunks = acc;
+  return TM_OK;
+}
+
+static int chunk_roots_of(const tm_tuning *t) {
+  int c = (t && t->chunk_roots > 0) ? t->chunk_roots : 256;
+  c = (c + 63) / 64 * 64;
+  return c;
+}
+
+static tm_status upload_staged(tm_graph *g, int nq) {
+  Scratch &s = g->scratch;
+  HIPTRY(hipMemcpyAsync(s.d_queries, s.h_queries.data(), sizeof(tm_query_dev) * nq, hipMemcpyHostToDevice,
+                        g->stream));
+  HIPTRY(hipMemcpyAsync(s.d_prefix, s.h_prefix.data(), sizeof(long long) * (nq + 1), hipMemcpyHostToDevice,
+                        g->stream));
+  return TM_OK;
+}
+
+static tm_status finish(tm_graph *g, int nq, uint64_t *counts) {
+  Scratch &s = g->scratch;
+  tm_sched hs;
+  HIPTRY(hipMemcpyAsync(&hs, s.d_sched, sizeof(hs), hipMemcpyDeviceToHost, g->stream));
+  HIPTRY(hipMemcpyAsync(counts, s.d_counts, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, g->stream));
+  HIPTRY(hipStreamSynchronize(g->stream));
+  if (hs.error) return fail(TM_ERR_INTERNAL, "kernel frame stack overflow");
+  return TM_OK;
+}
+
+tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                   int32_t beg, int32_t end, const tm_tuning *tune, uint64_t *count) {
+  if (!g || !msrc || !mdst || !count) return fail(TM_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(g->mtx);
+  HIPTRY(hipSetDevice(g->device));
+  const int cr = chunk_roots_of(tune);
+  long long chunks;
+  int maxM, maxN;
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, cr, &chunks, &maxM,
+                               &maxN);
+  if (st != TM_OK) return st;
+  if ((st = upload_staged(g, 1)) != TM_OK) return st;
+  if ((st = launch(g, 1, maxM, maxN, false, tune, chunks, cr)) != TM_OK) return st;
+  return finish(g, 1, count);
+}
+
+tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, const int32_t *msrc,
+                         const int32_t *mdst, const int32_t *deltas, const tm_tuning *tune, uint64_t *counts) {
+  if (!g || nq < 1 || !num_edges || !msrc || !mdst || !deltas || !counts) return fail(TM_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(g->mtx);
+  HIPTRY(hipSetDevice(g->device));
+  const int cr = chunk_roots_of(tune);
+  long long chunks;
+  int maxM, maxN;
+  tm_status st = stage_queries(g, nq, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas, nullptr, nullptr, cr,
+                               &chunks, &maxM, &maxN);
+  if (st != TM_OK) return st;
+  if ((st = upload_staged(g, nq)) != TM_OK) return st;
+  if ((st = launch(g, nq, maxM, maxN, false, tune, chunks, cr)) != TM_OK) return st;
+  return finish(g, nq, counts);
+}
+
+tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                  int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows, uint64_t *count,
+                  int64_t *rows_written) {
+  if (!g || !msrc || !mdst || !count || cap < 0 || (cap > 0 && !rows)) return fail(TM_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(g->mtx);
+  HIPTRY(hipSetDevice(g->device));
+  const int cr = chunk_roots_of(tune);
+  long long chunks;
+  int maxM, maxN;
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, cr, &chunks, &maxM,
+                               &maxN);
+  if (st != TM_OK) return st;
+  Scratch &s = g->scratch;
+  int32_t *d_rows = nullptr;
+  const size_t row_bytes = sizeof(int32_t) * (size_t)std::max<int64_t>(cap, 1) * num_edges;
+  HIPTRY(hipMalloc(&d_rows, row_bytes));
+  s.h_queries[0].rows = d_rows;
+  s.h_queries[0].cap = cap;
+  if ((st = upload_staged(g, 1)) != TM_OK) { (void)hipFree(d_rows); return st; }
+  if ((st = launch(g, 1, maxM, maxN, true, tune, chunks, cr)) != TM_OK) { (void)hipFree(d_rows); return st; }
+  st = finish(g, 1, count);
+  if (st == TM_OK) {
+    int64_t n = std::min<int64_t>((int64_t)*count, cap);
+    if (n > 0) {
+      hipError_t he = hipMemcpy(rows, d_rows, sizeof(int32_t) * (size_t)n * num_edges, hipMemcpyDeviceToHost);
+      if (he != hipSuccess) st = fail(TM_ERR_HIP, hipGetErrorString(he));
+    }
+    if (rows_written) *rows_written = n;
+  }
+  (void)hipFree(d_rows);
+  return st;
+}
+
+}  // extern "C"

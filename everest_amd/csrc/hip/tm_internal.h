@@ -1,0 +1,26 @@
+// tm_internal.h -- host-side state shared by the library's translation units.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "tm_abi.h"
+#include "tm_types.h"
+
+namespace tmh {
+
+// Per-graph staging area for query descriptors and results (device + host mirrors).
+struct Scratch {
+  int cap_q = 0;
+  tm_query_dev *d_queries = nullptr;
+  long long *d_prefix = nullptr;
+  unsigned long long *d_counts = nullptr;
+  int *d_stops = nullptr;
+  tm_sched *d_sched = nullptr;
+  std::vector<tm_query_dev> h_queries;
+  std::vector<long long> h_prefix;
+};
+
+tm_status set_error(tm_status s, const std::string &msg);
+tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_dev *md);
+
+}  // namespace tmh

@@ -1,0 +1,133 @@
+/*
+ * tm_abi.h -- C ABI of the MI355X temporal-motif query library (libtm_hip.so).
+ *
+ * Plain C types only: int32/int64/uint64 scalars, pointers to host arrays and
+ * an opaque graph handle.  Every entry point returns a tm_status; on failure
+ * tm_last_error() holds a message for the calling thread.  Device work runs
+ * on the library's own HIP stream for the graph's device and is complete when
+ * the call returns (the reference's launches are synchronous from the
+ * caller's view: a D2H copy ends each run, system/templates/Impl/result.snp).
+ *
+ * Semantics of a match (reference: plugins/GPUWorkerU1B.cu:10-263 and
+ * system/templates/common/ snippets): a strictly increasing tuple of positions
+ * p_0 < ... < p_{M-1} in the stably time-sorted edge list, an injective motif
+ * node -> graph node map, t(p_{M-1}) - t(p_0) <= delta (inclusive), root
+ * p_0 in [beg, end) with u != v.  Counts are exact uint64.
+ */
+#ifndef TM_ABI_H_
+#define TM_ABI_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TM_ABI_VERSION 1
+#define TM_MAX_MOTIF_EDGES 8
+#define TM_MAX_MOTIF_NODES 8
+
+typedef enum tm_status {
+  TM_OK = 0,
+  TM_ERR_ARG = 1,       /* bad argument (null pointer, range, size) */
+  TM_ERR_MOTIF = 2,     /* motif not prefix-connected, self-loop, too large */
+  TM_ERR_HIP = 3,       /* a HIP runtime call failed */
+  TM_ERR_NOMEM = 4,     /* device allocation failed or budget exceeded */
+  TM_ERR_INTERNAL = 5,  /* kernel-side invariant violated (result discarded) */
+  TM_ERR_NODEVICE = 6   /* no usable GPU */
+} tm_status;
+
+/* Opaque device-resident graph (one device). */
+typedef struct tm_graph tm_graph;
+
+/*
+ * Reference-layout host graph: the arrays of corelib::data::HostGraphData
+ * (include/data.h:174-186, filled by GraphDataLoader::createGraphData,
+ * src/core/gpu/data.cu:124-348).  Eg holds 3 ints (u, v, t) per edge, edges
+ * stably sorted by t, vertex ids renamed 0..V-1.
+ */
+typedef struct tm_host_graph {
+  int32_t num_edges;
+  int32_t num_vertices;
+  const int32_t *Eg;         /* 3 * num_edges */
+  const int32_t *inEdgesV;   /* num_edges: positions grouped by destination */
+  const int32_t *outEdgesV;  /* num_edges: positions grouped by source */
+  const int32_t *inEdgesR;   /* num_vertices + 1 */
+  const int32_t *outEdgesR;  /* num_vertices + 1 */
+  const int32_t *Euid;       /* num_edges: original line numbers */
+} tm_host_graph;
+
+/* Launch parameters; zero fields take the library defaults. */
+typedef struct tm_tuning {
+  int32_t waves_per_cu;  /* resident wavefronts per CU in the persistent grid */
+  int32_t chunk_roots;   /* roots per dispatched chunk, multiple of 64 */
+  int32_t scan_max;      /* last-edge ranges up to this size are scanned, larger ones use the pair index */
+  int32_t reserved[5];
+} tm_tuning;
+
+/* Library version (TM_ABI_VERSION) and last error of the calling thread. */
+int32_t tm_abi_version(void);
+const char *tm_last_error(void);
+
+/* Number of visible GPUs (0 when none; never fails). */
+int32_t tm_device_count(void);
+
+/*
+ * Upload a host graph to `device` and build the device-side indexes
+ * (packed times, neighbour arrays, per-pair index, time buckets).
+ * Replaces GraphDataManager::alloc + moveAsync + waitMove
+ * (src/core/gpu/data.cu:384-472) and the pointer binding of DeviceJobData
+ * (src/core/gpu/data.cu:496-529).
+ */
+tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **out);
+
+/* Release a graph's device memory (reference: DevicePtrDeleter, data.cu:364-370). */
+tm_status tm_graph_free(tm_graph *g);
+
+/* Device bytes held by a graph, and its edge / vertex counts. */
+tm_status tm_graph_info(const tm_graph *g, int64_t *device_bytes, int32_t *num_edges,
+                        int32_t *num_vertices, int32_t *device);
+
+/*
+ * Count the matches of one motif.  The motif is num_edges (src, dst) pairs
+ * over renamed node ids (first appearance, src before dst) in time order, the
+ * form HostMotifData holds (src/core/cpu/data.cc:411-420).  Roots are
+ * [beg, end); end < 0 means num_edges_graph - num_edges + 1
+ * (MineJob, src/core/cpu/data.cc:452-457).
+ * Replaces GPUWorker::run of a generated worker (system/templates/GPUWorker/run.snp,
+ * system/templates/Impl/top.snp; plugins/GPUWorkerU1B.cu:230-263).
+ */
+tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                   int32_t delta, int32_t beg, int32_t end, const tm_tuning *tune,
+                   uint64_t *count);
+
+/*
+ * Enumerate matches: writes up to `cap` rows of num_edges int32 original line
+ * numbers (Euid of p_0 .. p_{M-1}) to `rows` (host, cap * num_edges ints) and
+ * the number of matches found to *count.  When *count <= cap every match is
+ * written; beyond the cap, dispatch of new roots stops and which rows are
+ * returned depends on the schedule, as in the reference
+ * (system/tmine/KernelGen.py:509-528, system/templates/enumerate/func.snp).
+ * *rows_written = min(*count, cap).
+ */
+tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                  int32_t delta, int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune,
+                  int32_t *rows, uint64_t *count, int64_t *rows_written);
+
+/*
+ * Count many motif queries on one graph in a single persistent launch.
+ * Query q uses edges msrc/mdst[q * TM_MAX_MOTIF_EDGES ...], num_edges[q],
+ * deltas[q] and roots [0, E - num_edges[q] + 1).  counts[q] receives the
+ * exact count, equal to tm_count of the same query.  No reference
+ * counterpart: the reference runs one generated worker per query
+ * (system/batch-query.py:12-19).
+ */
+tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges,
+                         const int32_t *msrc, const int32_t *mdst, const int32_t *deltas,
+                         const tm_tuning *tune, uint64_t *counts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TM_ABI_H_ */
