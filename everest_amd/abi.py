@@ -15,7 +15,8 @@ MAX_E = 8
 
 TM_OK = 0
 STATUS = {0: "TM_OK", 1: "TM_ERR_ARG", 2: "TM_ERR_MOTIF", 3: "TM_ERR_HIP", 4: "TM_ERR_NOMEM",
-          5: "TM_ERR_INTERNAL", 6: "TM_ERR_NODEVICE"}
+          5: "TM_ERR_INTERNAL", 6: "TM_ERR_NODEVICE", 7: "TM_ERR_TIMEOUT"}
+TM_ERR_TIMEOUT = 7
 
 
 class TMError(RuntimeError):
@@ -32,7 +33,8 @@ class HostGraph(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_roots", ctypes.c_int32),
-                ("scan_max", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("scan_max", ctypes.c_int32), ("time_limit_ms", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 _lib = None
@@ -56,8 +58,9 @@ def lib():
         L.tm_enum.argtypes = [P, I32, P, P, I32, I32, I32, I64, ctypes.POINTER(Tuning), P, P,
                               ctypes.POINTER(I64)]
         L.tm_batch_count.argtypes = [P, I32, P, P, P, P, ctypes.POINTER(Tuning), P]
+        L.tm_last_stats.argtypes = [P, P, I32]
         for name in ("tm_graph_upload", "tm_graph_free", "tm_graph_info", "tm_count", "tm_enum",
-                     "tm_batch_count"):
+                     "tm_batch_count", "tm_last_stats"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -72,9 +75,10 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def tuning(waves_per_cu=0, chunk_roots=0, scan_max=0):
+def tuning(waves_per_cu=0, chunk_roots=0, scan_max=0, time_limit_ms=0):
     t = Tuning()
     t.waves_per_cu, t.chunk_roots, t.scan_max = waves_per_cu, chunk_roots, scan_max
+    t.time_limit_ms = time_limit_ms
     return t
 
 
@@ -109,6 +113,11 @@ class DeviceGraph:
         check(lib().tm_graph_info(self.handle, ctypes.byref(b), ctypes.byref(e), ctypes.byref(v),
                                   ctypes.byref(d)))
         return dict(device_bytes=b.value, num_edges=e.value, num_vertices=v.value, device=d.value)
+
+    def last_stats(self):
+        st = np.zeros(8, dtype=np.uint64)
+        check(lib().tm_last_stats(self.handle, _p(st), 8))
+        return dict(steps=int(st[0]), items=int(st[1]), roots=int(st[2]), leaves=int(st[3]), pushes=int(st[4]))
 
     def count(self, edges, delta, beg=0, end=-1, tune=None):
         src, dst = motif_arrays(edges)

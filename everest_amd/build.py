@@ -56,7 +56,7 @@ def build_hip():
 
 def build_core():
     out = os.path.join(LIB, "libeverest_core.so")
-    srcs = [os.path.join(CORE_SRC, f) for f in ("loader.cc", "motif.cc", "runtime.cc")]
+    srcs = [os.path.join(CORE_SRC, f) for f in ("loader.cc", "motif.cc", "runtime.cc", "gen.cc")]
     _run([CXX] + CXX_FLAGS + ["-shared", "-I" + INC] + srcs +
          ["-L" + LIB, "-ltm_hip", "-ldl", "-pthread"] + RPATH + ["-o", out])
     return out

@@ -10,6 +10,7 @@
 #include <sstream>
 
 #include "everest/data.h"
+#include "everest/gen.h"
 
 namespace py = pybind11;
 using namespace corelib;
@@ -228,4 +229,35 @@ PYBIND11_MODULE(corelib, m) {
 #undef BIND_MULTI
 
   m.def("allCount", &allCount);
+
+  auto gen = m.def_submodule("gen", "deterministic synthetic temporal graphs");
+  auto arrays3 = [](size_t E) {
+    return std::make_tuple(py::array_t<int>(E), py::array_t<int>(E), py::array_t<int>(E));
+  };
+  gen.def(
+      "zipf_uniform",
+      [arrays3](size_t N, size_t E, double alpha, int64_t span, uint64_t seed) {
+        auto out = arrays3(E);
+        int *U = std::get<0>(out).mutable_data(), *V = std::get<1>(out).mutable_data(),
+            *T = std::get<2>(out).mutable_data();
+        {
+          py::gil_scoped_release rel;
+          everest::gen::zipf_uniform(N, E, alpha, span, seed, U, V, T);
+        }
+        return out;
+      },
+      py::arg("N"), py::arg("E"), py::arg("alpha"), py::arg("span"), py::arg("seed"));
+  gen.def(
+      "powerlaw_bursty",
+      [arrays3](size_t N, size_t E, double alpha, double sigma, int64_t span, uint64_t seed) {
+        auto out = arrays3(E);
+        int *U = std::get<0>(out).mutable_data(), *V = std::get<1>(out).mutable_data(),
+            *T = std::get<2>(out).mutable_data();
+        {
+          py::gil_scoped_release rel;
+          everest::gen::powerlaw_bursty(N, E, alpha, sigma, span, seed, U, V, T);
+        }
+        return out;
+      },
+      py::arg("N"), py::arg("E"), py::arg("alpha"), py::arg("sigma"), py::arg("span"), py::arg("seed"));
 }

@@ -368,10 +368,30 @@ FeatureLoader::FeatureLoader(const EdgeListLoader *eloader, std::filesystem::pat
   const int *V = (const int *)eloader->getVPtr()->data();
   auto build = [&](std::vector<int> &fe, std::vector<int> &fvm, std::vector<int> &fvv) {
     fe = label_values(pathToRands_, E);
-    fvm.assign(U, U + E);
-    fvm.insert(fvm.end(), V, V + E);
-    std::sort(fvm.begin(), fvm.end());
-    fvm.erase(std::unique(fvm.begin(), fvm.end()), fvm.end());
+    // sorted unique vertex ids (dense flags when the id range is small)
+    fvm.clear();
+    if (E) {
+      int lo = INT32_MAX, hi = INT32_MIN;
+      for (size_t i = 0; i < E; i++) {
+        lo = std::min(lo, std::min(U[i], V[i]));
+        hi = std::max(hi, std::max(U[i], V[i]));
+      }
+      long long range = (long long)hi - lo + 1;
+      if (range <= (long long)std::max<size_t>(4 * E, (size_t)1 << 22)) {
+        std::vector<uint8_t> seen((size_t)range, 0);
+        for (size_t i = 0; i < E; i++) {
+          seen[(size_t)((long long)U[i] - lo)] = 1;
+          seen[(size_t)((long long)V[i] - lo)] = 1;
+        }
+        for (long long k = 0; k < range; k++)
+          if (seen[(size_t)k]) fvm.push_back((int)(lo + k));
+      } else {
+        fvm.assign(U, U + E);
+        fvm.insert(fvm.end(), V, V + E);
+        std::sort(fvm.begin(), fvm.end());
+        fvm.erase(std::unique(fvm.begin(), fvm.end()), fvm.end());
+      }
+    }
     fvv = label_values(pathToRands_, fvm.size());
   };
   if (eloader->inMemory()) {

@@ -54,6 +54,34 @@ __global__ void aux_kernel(const tm_edge *__restrict__ Eg, const int *__restrict
   ikey[i] = ((unsigned long long)(unsigned)ei.v << 32) | (unsigned)ei.u;
 }
 
+// slot of each position inside the list that holds it
+__global__ void self_kernel(const int *__restrict__ P, int E, int *__restrict__ self) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < E) self[P[s]] = s;
+}
+
+// outNext[p] = first slot of out(dst p) after p; inNext[p] = first slot of in(src p) after p
+__global__ void next_kernel(const tm_edge *__restrict__ Eg, int E, const int *__restrict__ outR,
+                            const int *__restrict__ outPos, const int *__restrict__ inR,
+                            const int *__restrict__ inPos, int *__restrict__ outNext, int *__restrict__ inNext) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  tm_edge e = Eg[p];
+  int lo = outR[e.v], hi = outR[e.v + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (outPos[mid] <= p) lo = mid + 1; else hi = mid;
+  }
+  outNext[p] = lo;
+  lo = inR[e.u];
+  hi = inR[e.u + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (inPos[mid] <= p) lo = mid + 1; else hi = mid;
+  }
+  inNext[p] = lo;
+}
+
 __global__ void low_word_kernel(const unsigned long long *__restrict__ k, int *__restrict__ out, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (int)(unsigned)(k[i] & 0xffffffffull);
@@ -74,7 +102,13 @@ __global__ void bucket_kernel(const int *__restrict__ T, int E, int tmin, int bs
 
 __global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) { s->next_chunk = 0; s->error = 0; }
+  if (i == 0) {
+    s->next_chunk = 0;
+    s->error = 0;
+    s->abort = 0;
+    s->t0 = (long long)wall_clock64();
+  }
+  if (i < 8) s->stats[i] = 0;
   if (i < n) { counts[i] = 0; stops[i] = 0; }
 }
 
@@ -92,11 +126,13 @@ struct tm_graph {
   int device = 0;
   int E = 0, V = 0;
   int num_cus = 0;
+  long long wall_rate_khz = 100000;  // wall_clock64 ticks per millisecond
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   int64_t bytes = 0;
   tm_graph_view view{};
   tmh::Scratch scratch;
+  unsigned long long last_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::mutex mtx;  // one launch at a time per graph handle
 };
 
@@ -130,13 +166,17 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
     md->a[j] = a;
     md->b[j] = b;
     md->mapped[j] = n;
+    const int pa = src[j - 1], pb = dst[j - 1];
     if (a < n && b < n) {
       md->kind[j] = TM_K_CLOSE;
+      md->smode[j] = (a == pa && b == pb) ? TM_S_PAIR_SELF : TM_S_SEARCH;
     } else if (a < n && b == n) {
       md->kind[j] = TM_K_NEW_DST;
+      md->smode[j] = a == pa ? TM_S_OUT_SELF : (a == pb ? TM_S_OUT_NEXT : TM_S_SEARCH);
       n++;
     } else if (b < n && a == n) {
       md->kind[j] = TM_K_NEW_SRC;
+      md->smode[j] = b == pb ? TM_S_IN_SELF : (b == pa ? TM_S_IN_NEXT : TM_S_SEARCH);
       n++;
     } else {
       return fail(TM_ERR_MOTIF, "motif is not prefix-connected with first-appearance node ids");
@@ -190,6 +230,8 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   a.total_chunks = total_chunks;
   a.frames = frames;
   a.scan_max = (tune && tune->scan_max > 0) ? tune->scan_max : 16;
+  a.limit_ticks = 0;
+  if (tune && tune->time_limit_ms > 0) a.limit_ticks = (long long)tune->time_limit_ms * (g->wall_rate_khz);
   a.sched = s.d_sched;
 
   hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, g->stream, s.d_sched,
@@ -282,6 +324,11 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   hipError_t he = hipGetDeviceProperties(&prop, device);
   if (he != hipSuccess) { delete g; return fail(TM_ERR_HIP, hipGetErrorString(he)); }
   g->num_cus = prop.multiProcessorCount;
+  {
+    int rate = 0;
+    if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, device) == hipSuccess && rate > 0)
+      g->wall_rate_khz = rate;
+  }
   he = hipStreamCreate(&g->stream);
   if (he != hipSuccess) { delete g; return fail(TM_ERR_HIP, hipGetErrorString(he)); }
 
@@ -311,6 +358,11 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   ALLOC(vw.inPairNbr, e4);
   ALLOC(vw.inPairPos, e4);
   ALLOC(vw.Euid, e4);
+  ALLOC(vw.outSelf, e4);
+  ALLOC(vw.inSelf, e4);
+  ALLOC(vw.outNext, e4);
+  ALLOC(vw.inNext, e4);
+  ALLOC(vw.outPairSelf, e4);
   const int tmin = Eg[0].t, tmax = Eg[E - 1].t;
   const long long span = (long long)tmax - tmin + 1;
   int bshift = ceil_log2(std::max(1ll, (span * 16) / std::max(1, E)));
@@ -362,6 +414,15 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
                                             end_bit, s));
   hipLaunchKernelGGL(low_word_kernel, dim3((E + 255) / 256), dim3(256), 0, s, skey, (int *)vw.inPairNbr, E);
   TMPCHK(hipGetLastError());
+  {
+    const dim3 gr((E + 255) / 256), bl(256);
+    hipLaunchKernelGGL(self_kernel, gr, bl, 0, s, vw.outPos, E, (int *)vw.outSelf);
+    hipLaunchKernelGGL(self_kernel, gr, bl, 0, s, vw.inPos, E, (int *)vw.inSelf);
+    hipLaunchKernelGGL(self_kernel, gr, bl, 0, s, vw.outPairPos, E, (int *)vw.outPairSelf);
+    hipLaunchKernelGGL(next_kernel, gr, bl, 0, s, vw.Eg, E, vw.outR, vw.outPos, vw.inR, vw.inPos,
+                       (int *)vw.outNext, (int *)vw.inNext);
+    TMPCHK(hipGetLastError());
+  }
   hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((nb + 1 + 255) / 256)), dim3(256), 0, s, vw.T, E, tmin,
                      bshift, nb, (int *)vw.bucket);
   TMPCHK(hipGetLastError());
@@ -380,6 +441,12 @@ tm_status tm_graph_free(tm_graph *g) {
   for (void *p : g->allocs) (void)hipFree(p);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
+  return TM_OK;
+}
+
+tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n) {
+  if (!g || !stats || n < 0) return fail(TM_ERR_ARG, "bad argument");
+  for (int i = 0; i < n && i < 8; i++) stats[i] = g->last_stats[i];
   return TM_OK;
 }
 
@@ -451,7 +518,9 @@ static tm_status finish(tm_graph *g, int nq, uint64_t *counts) {
   HIPTRY(hipMemcpyAsync(&hs, s.d_sched, sizeof(hs), hipMemcpyDeviceToHost, g->stream));
   HIPTRY(hipMemcpyAsync(counts, s.d_counts, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, g->stream));
   HIPTRY(hipStreamSynchronize(g->stream));
-  if (hs.error) return fail(TM_ERR_INTERNAL, "kernel frame stack overflow");
+  memcpy(g->last_stats, hs.stats, sizeof(g->last_stats));
+  if (hs.error & 4) return fail(TM_ERR_TIMEOUT, "query exceeded its time limit; counts discarded");
+  if (hs.error) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(hs.error) + ")");
   return TM_OK;
 }
 

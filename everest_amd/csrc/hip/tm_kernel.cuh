@@ -108,93 +108,136 @@ __device__ __forceinline__ int pend_of(const tm_graph_view &g, long long tl) {
   return upper_bound_i(g.T, lo, hi, (int)tl);
 }
 
-// number of entries of pair segment (x -> y) inside one node's pair-sorted list
-// [lo, hi) whose positions lie in (p, pend)
+// first index in [lo, hi) with a[i] >= x, probing outward from lo
+// (exponential search: cheap when the answer is near lo, log otherwise)
+__device__ __forceinline__ int gallop_lb(const int *__restrict__ a, int lo, int hi, int x) {
+  if (lo >= hi || a[lo] >= x) return lo;
+  int prev = lo, step = 1;
+  for (;;) {
+    int nxt = prev + step;
+    if (nxt >= hi) return lower_bound_i(a, prev + 1, hi, x);
+    if (a[nxt] >= x) return lower_bound_i(a, prev + 1, nxt, x);
+    prev = nxt;
+    step <<= 1;
+  }
+}
+
+// first index in [lo, hi) with a[i] > x, probing outward from lo
+__device__ __forceinline__ int gallop_ub(const int *__restrict__ a, int lo, int hi, int x) {
+  if (lo >= hi || a[lo] > x) return lo;
+  int prev = lo, step = 1;
+  for (;;) {
+    int nxt = prev + step;
+    if (nxt >= hi) return upper_bound_i(a, prev + 1, hi, x);
+    if (a[nxt] > x) return upper_bound_i(a, prev + 1, nxt, x);
+    prev = nxt;
+    step <<= 1;
+  }
+}
+
+// Entries of the pair segment key inside one node's (neighbour, position)-
+// sorted list [lo, hi) whose positions lie in (p, pend).
 __device__ __forceinline__ int pair_count(const int *__restrict__ pnbr, const int *__restrict__ ppos,
-                                          int lo, int hi, int y, int p, int pend) {
-  int s0 = lower_bound_i(pnbr, lo, hi, y);
-  if (s0 >= hi || pnbr[s0] != y) return 0;
-  int s1 = upper_bound_i(pnbr, s0 + 1, hi, y);
+                                          int lo, int hi, int key, int p, int pend) {
+  int s0 = lower_bound_i(pnbr, lo, hi, key);
+  if (s0 >= hi || pnbr[s0] != key) return 0;
+  int s1 = gallop_ub(pnbr, s0 + 1, hi, key);
   int c0 = upper_bound_i(ppos, s0, s1, p);
-  int c1 = lower_bound_i(ppos, c0, s1, pend);
+  int c1 = gallop_lb(ppos, c0, s1, pend);
   return c1 - c0;
 }
 
-// Candidate slot range feeding motif edge j, given the mapping after edge j-1.
-// dir (CLOSE only): 0 = slots index the out-pair index of m[a], 1 = the in-pair
-// index of m[b].
-template <int MAXN>
-__device__ __forceinline__ void child_range(const tm_graph_view &g, const tm_motif_dev &md, int j,
-                                            const int (&m)[MAXN], int p, int pend,
-                                            int &cb, int &ce, int &dir) {
-  const int kind = md.kind[j];
-  dir = 0;
-  if (kind == TM_K_NEW_DST) {
-    int x = sel(m, md.a[j]);
-    int lo = g.outR[x], hi = g.outR[x + 1];
-    cb = upper_bound_i(g.outPos, lo, hi, p);
-    ce = lower_bound_i(g.outPos, cb, hi, pend);
-  } else if (kind == TM_K_NEW_SRC) {
-    int y = sel(m, md.b[j]);
-    int lo = g.inR[y], hi = g.inR[y + 1];
-    cb = upper_bound_i(g.inPos, lo, hi, p);
-    ce = lower_bound_i(g.inPos, cb, hi, pend);
-  } else {
-    int x = sel(m, md.a[j]), y = sel(m, md.b[j]);
-    int olo = g.outR[x], ohi = g.outR[x + 1];
-    int ilo = g.inR[y], ihi = g.inR[y + 1];
-    const int *pp;
-    int s0, s1;
-    if (ohi - olo <= ihi - ilo) {
-      s0 = lower_bound_i(g.outPairNbr, olo, ohi, y);
-      s1 = upper_bound_i(g.outPairNbr, s0, ohi, y);
-      pp = g.outPairPos;
-    } else {
-      s0 = lower_bound_i(g.inPairNbr, ilo, ihi, x);
-      s1 = upper_bound_i(g.inPairNbr, s0, ihi, x);
-      pp = g.inPairPos;
-      dir = 1;
-    }
-    cb = upper_bound_i(pp, s0, s1, p);
-    ce = lower_bound_i(pp, cb, s1, pend);
-  }
+// Edges x -> y with positions in (p, pend), searched in the shorter of out(x)
+// and in(y).
+__device__ __forceinline__ int edge_count(const tm_graph_view &g, int x, int y, int p, int pend) {
+  int olo = g.outR[x], ohi = g.outR[x + 1];
+  int ilo = g.inR[y], ihi = g.inR[y + 1];
+  if (ohi - olo <= ihi - ilo) return pair_count(g.outPairNbr, g.outPairPos, olo, ohi, y, p, pend);
+  return pair_count(g.inPairNbr, g.inPairPos, ilo, ihi, x, p, pend);
 }
 
-// Matches of the final motif edge j given the mapping through edge j-1,
-// counted without enumerating them.
+// Candidate list of motif edge j: slots [lo, hi) of `pos` (and `nbr` for new
+// nodes) whose positions are > p, the position matched for edge j-1.
+// dir (CLOSE only): 0 = out-pair index of m[a], 1 = in-pair index of m[b].
+struct CandList {
+  const int *pos;
+  const int *nbr;
+  int lo, hi, dir;
+};
+
+template <int MAXN>
+__device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_motif_dev &md, int j,
+                                               const int (&m)[MAXN], int p) {
+  CandList L;
+  L.dir = 0;
+  const int kind = md.kind[j];
+  const int mode = md.smode[j];
+  if (kind == TM_K_NEW_DST) {
+    const int x = sel(m, md.a[j]);
+    L.pos = g.outPos;
+    L.nbr = g.outNbr;
+    L.hi = g.outR[x + 1];
+    if (mode == TM_S_OUT_SELF) L.lo = g.outSelf[p] + 1;
+    else if (mode == TM_S_OUT_NEXT) L.lo = g.outNext[p];
+    else L.lo = upper_bound_i(g.outPos, g.outR[x], L.hi, p);
+  } else if (kind == TM_K_NEW_SRC) {
+    const int y = sel(m, md.b[j]);
+    L.pos = g.inPos;
+    L.nbr = g.inNbr;
+    L.hi = g.inR[y + 1];
+    if (mode == TM_S_IN_SELF) L.lo = g.inSelf[p] + 1;
+    else if (mode == TM_S_IN_NEXT) L.lo = g.inNext[p];
+    else L.lo = upper_bound_i(g.inPos, g.inR[y], L.hi, p);
+  } else {
+    const int x = sel(m, md.a[j]), y = sel(m, md.b[j]);
+    L.nbr = nullptr;
+    if (mode == TM_S_PAIR_SELF) {
+      // the same pair as edge j-1: continue right after p inside its segment
+      L.pos = g.outPairPos;
+      L.lo = g.outPairSelf[p] + 1;
+      L.hi = gallop_ub(g.outPairNbr, L.lo, g.outR[x + 1], y);
+    } else {
+      int olo = g.outR[x], ohi = g.outR[x + 1];
+      int ilo = g.inR[y], ihi = g.inR[y + 1];
+      int s0, s1;
+      if (ohi - olo <= ihi - ilo) {
+        s0 = lower_bound_i(g.outPairNbr, olo, ohi, y);
+        s1 = gallop_ub(g.outPairNbr, s0, ohi, y);
+        L.pos = g.outPairPos;
+      } else {
+        s0 = lower_bound_i(g.inPairNbr, ilo, ihi, x);
+        s1 = gallop_ub(g.inPairNbr, s0, ihi, x);
+        L.pos = g.inPairPos;
+        L.dir = 1;
+      }
+      L.lo = upper_bound_i(L.pos, s0, s1, p);
+      L.hi = s1;
+    }
+  }
+  return L;
+}
+
+// Matches of the final motif edge j given the mapping through edge j-1 and the
+// list from cand_start, counted without enumerating them.
 template <int MAXN>
 __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g, const tm_motif_dev &md,
-                                                         int j, const int (&m)[MAXN], int p, int pend,
-                                                         int scan_max) {
-  const int kind = md.kind[j];
-  if (kind == TM_K_CLOSE) {
-    int x = sel(m, md.a[j]), y = sel(m, md.b[j]);
-    int olo = g.outR[x], ohi = g.outR[x + 1];
-    int ilo = g.inR[y], ihi = g.inR[y + 1];
-    if (ohi - olo <= ihi - ilo) return pair_count(g.outPairNbr, g.outPairPos, olo, ohi, y, p, pend);
-    return pair_count(g.inPairNbr, g.inPairPos, ilo, ihi, x, p, pend);
-  }
-  const bool out = (kind == TM_K_NEW_DST);
-  const int base = out ? sel(m, md.a[j]) : sel(m, md.b[j]);
-  const int *R = out ? g.outR : g.inR;
-  const int *pos = out ? g.outPos : g.inPos;
-  const int *nbr = out ? g.outNbr : g.inNbr;
-  int lo = R[base], hi = R[base + 1];
-  int b0 = upper_bound_i(pos, lo, hi, p);
-  int b1 = lower_bound_i(pos, b0, hi, pend);
-  int c = b1 - b0;
-  if (c <= 0) return 0;
+                                                         int j, const int (&m)[MAXN], const CandList &L,
+                                                         int p, int pend, int scan_max) {
+  const int b1 = gallop_lb(L.pos, L.lo, L.hi, pend);
+  int c = b1 - L.lo;
+  if (c <= 0 || md.kind[j] == TM_K_CLOSE) return c > 0 ? (unsigned long long)c : 0ull;
   const int nm = md.mapped[j];
   if (c <= scan_max) {
     int bad = 0;
-    for (int s = b0; s < b1; s++) bad += in_first(m, nm, nbr[s]) ? 1 : 0;
+    for (int s = L.lo; s < b1; s++) bad += in_first(m, nm, L.nbr[s]) ? 1 : 0;
     return (unsigned long long)(c - bad);
   }
-  const int *pn = out ? g.outPairNbr : g.inPairNbr;
-  const int *pq = out ? g.outPairPos : g.inPairPos;
+  // injectivity: remove candidates whose new endpoint is already mapped
+  const bool out = md.kind[j] == TM_K_NEW_DST;
+  const int base = out ? sel(m, md.a[j]) : sel(m, md.b[j]);
 #pragma unroll
   for (int i = 0; i < MAXN; i++)
-    if (i < nm) c -= pair_count(pn, pq, lo, hi, m[i], p, pend);
+    if (i < nm) c -= out ? edge_count(g, base, m[i], p, pend) : edge_count(g, m[i], base, p, pend);
   return (unsigned long long)c;
 }
 
@@ -221,6 +264,8 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   unsigned long long cnt = 0;
+  unsigned long long st_steps = 0, st_items = 0, st_roots = 0, st_leaves = 0, st_push = 0;
+  unsigned iter = 0;
   int qcur = -1;
   int top = 0;
   int chunk_left = 0;      // 64-root groups left in the current chunk
@@ -228,6 +273,14 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   int root_end = 0;
 
   for (;;) {
+    if ((++iter & 63u) == 0u) {  // deadline check (wall clock, uniform)
+      if (args.limit_ticks > 0 && lane == 0 && (long long)wall_clock64() - sched->t0 > args.limit_ticks)
+        atomicOr(&sched->abort, 1);
+      if (uni(*((volatile int *)&sched->abort))) {
+        if (lane == 0) atomicOr(&sched->error, 4);
+        break;
+      }
+    }
     if (top == 0) {
       // ---------------------------------------------------- root dispatch
       if (chunk_left == 0) {
@@ -266,6 +319,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       const int r = next_root + lane;
       next_root += WAVE;
       chunk_left--;
+      st_roots += (r < root_end) ? 1 : 0;
       bool ok = r < root_end;
       int u = 0, v = 0, t = 0;
       if (ok) {
@@ -273,24 +327,32 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         u = e.u; v = e.v; t = e.t;
         ok = (u != v);
       }
-      int pend = 0;
-      if (ok) {
-        pend = pend_of(g, (long long)t + q.delta);
-        ok = (pend - r - 1) >= (M - 1);
-      }
       int m[MAXN];
 #pragma unroll
       for (int i = 0; i < MAXN; i++) m[i] = 0;
       m[0] = u; m[1] = v;
+      // candidates of edge 1; most roots stop at the first candidate's time
+      CandList L{g.outPos, g.outNbr, 0, 0, 0};
+      const long long tl = (long long)t + q.delta;
+      if (ok) {
+        L = cand_start(g, md, 1, m, r);
+        ok = L.lo < L.hi && (long long)g.T[L.pos[L.lo]] <= tl;
+      }
+      int pend = 0;
+      if (ok) {
+        pend = pend_of(g, tl);
+        ok = (pend - r - 1) >= (M - 1);
+      }
       const int last = ENUM ? M - 1 : M - 2;  // last motif edge handled per candidate
       if (last == 0) {
         // count mode, M == 2: edge 1 is the final edge
-        if (ok) cnt += leaf_count(g, md, 1, m, r, pend, args.scan_max);
+        if (ok) { cnt += leaf_count(g, md, 1, m, L, r, pend, args.scan_max); st_leaves++; }
         continue;
       }
-      int cb = 0, ce = 0, dir = 0;
+      int cb = 0, ce = 0, dir = L.dir;
       if (ok) {
-        child_range(g, md, 1, m, r, pend, cb, ce, dir);
+        cb = L.lo;
+        ce = gallop_lb(L.pos, L.lo, L.hi, pend);
         ok = ce > cb;
       }
       unsigned long long bm = __ballot(ok);
@@ -345,6 +407,8 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     }
     const bool has = lane < total;
     const int myf = top - 1 - rank;
+    st_steps++;
+    st_items += has ? 1 : 0;
     int slot = 0, p = 0, pend = 0, dir = 0;
     int m[MAXN];
     int e[ENUM ? MAXE : 1];
@@ -401,7 +465,11 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
           if ((long long)w + 1 >= q.cap) *((volatile int *)q.stop) = 1;
         }
       } else {
-        if (valid) cnt += leaf_count(g, md, j + 1, m, pos, pend, args.scan_max);
+        if (valid) {
+          const CandList LL = cand_start(g, md, j + 1, m, pos);
+          cnt += leaf_count(g, md, j + 1, m, LL, pos, pend, args.scan_max);
+          st_leaves++;
+        }
       }
       top = ntop;
       __syncthreads();
@@ -409,7 +477,10 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     }
     int cb = 0, ce = 0, cdir = 0;
     if (valid) {
-      child_range(g, md, j + 1, m, pos, pend, cb, ce, cdir);
+      const CandList LL = cand_start(g, md, j + 1, m, pos);
+      cb = LL.lo;
+      cdir = LL.dir;
+      ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend);
       valid = ce > cb;
     }
     const unsigned long long bm = __ballot(valid);
@@ -427,6 +498,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         f_e[j * FR + w] = pos;
       }
     }
+    st_push += valid ? 1 : 0;
     top = ntop + n;
     __syncthreads();
   }
@@ -434,6 +506,16 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   if (!ENUM && qcur >= 0) {
     unsigned long long s = wave_sum64(cnt);
     if (lane == 0 && s) atomicAdd(args.queries[qcur].count, s);
+  }
+  // work counters: one atomic per counter per wavefront
+  unsigned long long w0 = wave_sum64(st_items), w1 = wave_sum64(st_roots), w2 = wave_sum64(st_leaves),
+                     w3 = wave_sum64(st_push);
+  if (lane == 0) {
+    atomicAdd(&sched->stats[TM_STAT_STEPS], st_steps);
+    atomicAdd(&sched->stats[TM_STAT_ITEMS], w0);
+    atomicAdd(&sched->stats[TM_STAT_ROOTS], w1);
+    atomicAdd(&sched->stats[TM_STAT_LEAVES], w2);
+    atomicAdd(&sched->stats[TM_STAT_PUSHES], w3);
   }
 }
 

@@ -14,6 +14,17 @@ enum {
   TM_K_CLOSE = 2     // both mapped: candidates = the (m[a] -> m[b]) pair segment
 };
 
+// Where the candidate list of motif edge j starts, given the position p matched
+// for edge j-1 (fixed per motif edge by which node the list hangs off).
+enum {
+  TM_S_SEARCH = 0,     // binary search for the first position > p
+  TM_S_OUT_SELF = 1,   // list out(src of p):  outSelf[p] + 1
+  TM_S_OUT_NEXT = 2,   // list out(dst of p):  outNext[p]
+  TM_S_IN_SELF = 3,    // list in(dst of p):   inSelf[p] + 1
+  TM_S_IN_NEXT = 4,    // list in(src of p):   inNext[p]
+  TM_S_PAIR_SELF = 5   // same pair as edge j-1: outPairSelf[p] + 1
+};
+
 typedef struct { int u, v, t; } tm_edge;  // reference TemporalEdge (include/data.h:27-29)
 
 typedef struct {
@@ -23,6 +34,7 @@ typedef struct {
   int a[TM_MAXE];       // motif source node of edge j
   int b[TM_MAXE];       // motif destination node of edge j
   int mapped[TM_MAXE];  // nodes mapped before edge j (a new node gets this index)
+  int smode[TM_MAXE];   // TM_S_* start rule of edge j's candidate list
 } tm_motif_dev;
 
 // Device-resident graph: reference arrays plus the indexes built at upload.
@@ -34,6 +46,11 @@ typedef struct {
   const int *inR, *inPos, *inNbr;     // in CSR: offsets, positions (inEdgesV), sources
   const int *outPairNbr, *outPairPos; // per node, out-list re-sorted by (dst, position)
   const int *inPairNbr, *inPairPos;   // per node, in-list re-sorted by (src, position)
+  const int *outSelf;     // slot of position p in out(src of p)
+  const int *inSelf;      // slot of position p in in(dst of p)
+  const int *outNext;     // first slot of out(dst of p) with position > p
+  const int *inNext;      // first slot of in(src of p) with position > p
+  const int *outPairSelf; // slot of position p in the out-pair index of src of p
   const int *bucket;      // bucket[k] = first position with T >= tmin + (k << bshift)
   int tmin, bshift;
   long long nbuckets;
@@ -53,9 +70,13 @@ typedef struct {
 
 typedef struct {
   unsigned long long next_chunk;
-  int error;
-  int pad[13];
+  int error;        // bit 0: frame stack overflow, bit 1: empty frame, bit 2: deadline passed
+  int abort;        // set once the deadline passed: every wavefront stops
+  long long t0;     // wall clock at launch (set by the reset kernel)
+  unsigned long long stats[8];  // work counters (see TM_STAT_*)
 } tm_sched;
+
+enum { TM_STAT_STEPS = 0, TM_STAT_ITEMS = 1, TM_STAT_ROOTS = 2, TM_STAT_LEAVES = 3, TM_STAT_PUSHES = 4 };
 
 typedef struct {
   tm_graph_view g;
@@ -66,5 +87,6 @@ typedef struct {
   long long total_chunks;
   int frames;                 // LDS frame capacity per wavefront
   int scan_max;               // last-edge ranges up to this size are scanned
+  long long limit_ticks;      // wall-clock budget of the launch (0 = none)
   tm_sched *sched;
 } tm_kernel_args;

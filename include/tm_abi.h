@@ -34,7 +34,8 @@ typedef enum tm_status {
   TM_ERR_HIP = 3,       /* a HIP runtime call failed */
   TM_ERR_NOMEM = 4,     /* device allocation failed or budget exceeded */
   TM_ERR_INTERNAL = 5,  /* kernel-side invariant violated (result discarded) */
-  TM_ERR_NODEVICE = 6   /* no usable GPU */
+  TM_ERR_NODEVICE = 6,  /* no usable GPU */
+  TM_ERR_TIMEOUT = 7    /* the launch passed tm_tuning.time_limit_ms; results discarded */
 } tm_status;
 
 /* Opaque device-resident graph (one device). */
@@ -62,7 +63,8 @@ typedef struct tm_tuning {
   int32_t waves_per_cu;  /* resident wavefronts per CU in the persistent grid */
   int32_t chunk_roots;   /* roots per dispatched chunk, multiple of 64 */
   int32_t scan_max;      /* last-edge ranges up to this size are scanned, larger ones use the pair index */
-  int32_t reserved[5];
+  int32_t time_limit_ms; /* abort the launch after this wall time on the device (0 = no limit) */
+  int32_t reserved[4];
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */
@@ -83,6 +85,13 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
 
 /* Release a graph's device memory (reference: DevicePtrDeleter, data.cu:364-370). */
 tm_status tm_graph_free(tm_graph *g);
+
+/*
+ * Work counters of the graph's last launch: [0] wavefront steps, [1] candidate
+ * items examined, [2] roots dispatched, [3] last-edge range computations,
+ * [4] frames pushed.  No reference counterpart (instrumentation).
+ */
+tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n);
 
 /* Device bytes held by a graph, and its edge / vertex counts. */
 tm_status tm_graph_info(const tm_graph *g, int64_t *device_bytes, int32_t *num_edges,

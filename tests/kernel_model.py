@@ -52,7 +52,19 @@ def build_view(g):
     nb = (span + (1 << bshift) - 1) >> bshift
     keys = tmin + (np.arange(nb + 1, dtype=np.int64) << bshift)
     bucket = np.searchsorted(T.astype(np.int64), keys, side="left")
+    outPairPos = outPos[o]
+    outSelf = np.empty(E, dtype=np.int64); outSelf[outPos] = np.arange(E)
+    inSelf = np.empty(E, dtype=np.int64); inSelf[inPos] = np.arange(E)
+    outPairSelf = np.empty(E, dtype=np.int64); outPairSelf[outPairPos] = np.arange(E)
+    outR, inR = g["outR"], g["inR"]
+    outNext = np.array([outR[Eg[p, 1]] + np.searchsorted(outPos[outR[Eg[p, 1]]:outR[Eg[p, 1] + 1]], p, side="right")
+                        for p in range(E)], dtype=np.int64)
+    inNext = np.array([inR[Eg[p, 0]] + np.searchsorted(inPos[inR[Eg[p, 0]]:inR[Eg[p, 0] + 1]], p, side="right")
+                       for p in range(E)], dtype=np.int64)
     return dict(E=E, V=V, Eg=Eg, T=Arr(T, "T"),
+                outSelf=Arr(outSelf, "outSelf"), inSelf=Arr(inSelf, "inSelf"),
+                outNext=Arr(outNext, "outNext"), inNext=Arr(inNext, "inNext"),
+                outPairSelf=Arr(outPairSelf, "outPairSelf"),
                 outR=Arr(g["outR"], "outR"), outPos=Arr(outPos, "outPos"), outNbr=Arr(outNbr, "outNbr"),
                 inR=Arr(g["inR"], "inR"), inPos=Arr(inPos, "inPos"), inNbr=Arr(inNbr, "inNbr"),
                 outPairNbr=Arr(outNbr[o], "outPairNbr"), outPairPos=Arr(outPos[o], "outPairPos"),
@@ -61,27 +73,34 @@ def build_view(g):
                 Euid=Arr(g["Euid"], "Euid"))
 
 
+S_SEARCH, S_OUT_SELF, S_OUT_NEXT, S_IN_SELF, S_IN_NEXT, S_PAIR_SELF = range(6)
+
+
 def build_motif(edges):
-    """tmh::build_motif: kinds / node slots per edge from renamed (a, b) pairs."""
+    """tmh::build_motif: kinds / node slots / start rules per edge."""
     M = len(edges)
     assert edges[0][0] == 0 and edges[0][1] == 1
-    kind, A, B, mapped = [0] * M, [0] * M, [0] * M, [0] * M
+    kind, A, B, mapped, smode = [0] * M, [0] * M, [0] * M, [0] * M, [0] * M
     n = 2
     for j in range(1, M):
         a, b = edges[j][0], edges[j][1]
+        pa, pb = edges[j - 1][0], edges[j - 1][1]
         assert a != b
         A[j], B[j], mapped[j] = a, b, n
         if a < n and b < n:
             kind[j] = K_CLOSE
+            smode[j] = S_PAIR_SELF if (a, b) == (pa, pb) else S_SEARCH
         elif a < n and b == n:
             kind[j] = K_NEW_DST
+            smode[j] = S_OUT_SELF if a == pa else (S_OUT_NEXT if a == pb else S_SEARCH)
             n += 1
         elif b < n and a == n:
             kind[j] = K_NEW_SRC
+            smode[j] = S_IN_SELF if b == pb else (S_IN_NEXT if b == pa else S_SEARCH)
             n += 1
         else:
             raise ValueError("not prefix-connected")
-    return dict(M=M, n=n, kind=kind, a=A, b=B, mapped=mapped)
+    return dict(M=M, n=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode)
 
 
 def lower_bound(a, lo, hi, x):
@@ -114,71 +133,106 @@ def pend_of(g, tl):
     return upper_bound(g["T"], lo, hi, tl)
 
 
-def pair_count(pn, pp, lo, hi, y, p, pend):
-    s0 = lower_bound(pn, lo, hi, y)
-    if s0 >= hi or pn[s0] != y:
+def gallop_lb(a, lo, hi, x):
+    if lo >= hi or a[lo] >= x:
+        return lo
+    prev, step = lo, 1
+    while True:
+        nxt = prev + step
+        if nxt >= hi:
+            return lower_bound(a, prev + 1, hi, x)
+        if a[nxt] >= x:
+            return lower_bound(a, prev + 1, nxt, x)
+        prev = nxt
+        step <<= 1
+
+
+def gallop_ub(a, lo, hi, x):
+    if lo >= hi or a[lo] > x:
+        return lo
+    prev, step = lo, 1
+    while True:
+        nxt = prev + step
+        if nxt >= hi:
+            return upper_bound(a, prev + 1, hi, x)
+        if a[nxt] > x:
+            return upper_bound(a, prev + 1, nxt, x)
+        prev = nxt
+        step <<= 1
+
+
+def pair_count(pn, pp, lo, hi, key, p, pend):
+    s0 = lower_bound(pn, lo, hi, key)
+    if s0 >= hi or pn[s0] != key:
         return 0
-    s1 = upper_bound(pn, s0 + 1, hi, y)
+    s1 = gallop_ub(pn, s0 + 1, hi, key)
     c0 = upper_bound(pp, s0, s1, p)
-    c1 = lower_bound(pp, c0, s1, pend)
+    c1 = gallop_lb(pp, c0, s1, pend)
     return c1 - c0
 
 
-def child_range(g, md, j, m, p, pend):
-    k = md["kind"][j]
+def edge_count(g, x, y, p, pend):
+    olo, ohi = g["outR"][x], g["outR"][x + 1]
+    ilo, ihi = g["inR"][y], g["inR"][y + 1]
+    if ohi - olo <= ihi - ilo:
+        return pair_count(g["outPairNbr"], g["outPairPos"], olo, ohi, y, p, pend)
+    return pair_count(g["inPairNbr"], g["inPairPos"], ilo, ihi, x, p, pend)
+
+
+def cand_start(g, md, j, m, p):
+    """Returns (pos, nbr, lo, hi, dir) as the kernel's CandList."""
+    k, mode = md["kind"][j], md["smode"][j]
     if k == K_NEW_DST:
         x = m[md["a"][j]]
-        lo, hi = g["outR"][x], g["outR"][x + 1]
-        cb = upper_bound(g["outPos"], lo, hi, p)
-        return cb, lower_bound(g["outPos"], cb, hi, pend), 0
+        hi = g["outR"][x + 1]
+        if mode == S_OUT_SELF:
+            lo = g["outSelf"][p] + 1
+        elif mode == S_OUT_NEXT:
+            lo = g["outNext"][p]
+        else:
+            lo = upper_bound(g["outPos"], g["outR"][x], hi, p)
+        return g["outPos"], g["outNbr"], lo, hi, 0
     if k == K_NEW_SRC:
         y = m[md["b"][j]]
-        lo, hi = g["inR"][y], g["inR"][y + 1]
-        cb = upper_bound(g["inPos"], lo, hi, p)
-        return cb, lower_bound(g["inPos"], cb, hi, pend), 0
+        hi = g["inR"][y + 1]
+        if mode == S_IN_SELF:
+            lo = g["inSelf"][p] + 1
+        elif mode == S_IN_NEXT:
+            lo = g["inNext"][p]
+        else:
+            lo = upper_bound(g["inPos"], g["inR"][y], hi, p)
+        return g["inPos"], g["inNbr"], lo, hi, 0
     x, y = m[md["a"][j]], m[md["b"][j]]
+    if mode == S_PAIR_SELF:
+        lo = g["outPairSelf"][p] + 1
+        hi = gallop_ub(g["outPairNbr"], lo, g["outR"][x + 1], y)
+        return g["outPairPos"], None, lo, hi, 0
     olo, ohi = g["outR"][x], g["outR"][x + 1]
     ilo, ihi = g["inR"][y], g["inR"][y + 1]
     if ohi - olo <= ihi - ilo:
         s0 = lower_bound(g["outPairNbr"], olo, ohi, y)
-        s1 = upper_bound(g["outPairNbr"], s0, ohi, y)
-        pp, d = g["outPairPos"], 0
+        s1 = gallop_ub(g["outPairNbr"], s0, ohi, y)
+        pos, d = g["outPairPos"], 0
     else:
         s0 = lower_bound(g["inPairNbr"], ilo, ihi, x)
-        s1 = upper_bound(g["inPairNbr"], s0, ihi, x)
-        pp, d = g["inPairPos"], 1
-    cb = upper_bound(pp, s0, s1, p)
-    return cb, lower_bound(pp, cb, s1, pend), d
+        s1 = gallop_ub(g["inPairNbr"], s0, ihi, x)
+        pos, d = g["inPairPos"], 1
+    return pos, None, upper_bound(pos, s0, s1, p), s1, d
 
 
-def leaf_count(g, md, j, m, p, pend, scan_max):
-    k = md["kind"][j]
-    if k == K_CLOSE:
-        x, y = m[md["a"][j]], m[md["b"][j]]
-        olo, ohi = g["outR"][x], g["outR"][x + 1]
-        ilo, ihi = g["inR"][y], g["inR"][y + 1]
-        if ohi - olo <= ihi - ilo:
-            return pair_count(g["outPairNbr"], g["outPairPos"], olo, ohi, y, p, pend)
-        return pair_count(g["inPairNbr"], g["inPairPos"], ilo, ihi, x, p, pend)
-    out = k == K_NEW_DST
-    base = m[md["a"][j]] if out else m[md["b"][j]]
-    R = g["outR"] if out else g["inR"]
-    pos = g["outPos"] if out else g["inPos"]
-    nbr = g["outNbr"] if out else g["inNbr"]
-    lo, hi = R[base], R[base + 1]
-    b0 = upper_bound(pos, lo, hi, p)
-    b1 = lower_bound(pos, b0, hi, pend)
-    c = b1 - b0
-    if c <= 0:
-        return 0
+def leaf_count(g, md, j, m, L, p, pend, scan_max):
+    pos, nbr, lo, hi, _ = L
+    b1 = gallop_lb(pos, lo, hi, pend)
+    c = b1 - lo
+    if c <= 0 or md["kind"][j] == K_CLOSE:
+        return max(c, 0)
     nm = md["mapped"][j]
     if c <= scan_max:
-        bad = sum(1 for s in range(b0, b1) if nbr[s] in m[:nm])
-        return c - bad
-    pn = g["outPairNbr"] if out else g["inPairNbr"]
-    pq = g["outPairPos"] if out else g["inPairPos"]
+        return c - sum(1 for s in range(lo, b1) if nbr[s] in m[:nm])
+    out = md["kind"][j] == K_NEW_DST
+    base = m[md["a"][j]] if out else m[md["b"][j]]
     for i in range(nm):
-        c -= pair_count(pn, pq, lo, hi, m[i], p, pend)
+        c -= edge_count(g, base, m[i], p, pend) if out else edge_count(g, m[i], base, p, pend)
     return c
 
 
@@ -211,15 +265,19 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                 u, v, t = (int(x) for x in Eg[r])
                 if u == v:
                     continue
+                m = [0] * MAXN
+                m[0], m[1] = u, v
+                L = cand_start(g, md, 1, m, r)
+                if not (L[2] < L[3] and g["T"][L[0][L[2]]] <= t + delta):
+                    continue
                 pend = pend_of(g, t + delta)
                 if pend - r - 1 < M - 1:
                     continue
-                m = [0] * MAXN
-                m[0], m[1] = u, v
                 if last == 0:
-                    cnt += leaf_count(g, md, 1, m, r, pend, scan_max)
+                    cnt += leaf_count(g, md, 1, m, L, r, pend, scan_max)
                     continue
-                cb, ce, d = child_range(g, md, 1, m, r, pend)
+                cb, d = L[2], L[4]
+                ce = gallop_lb(L[0], L[2], L[3], pend)
                 if ce > cb:
                     pushes.append(dict(cur=cb, end=ce, p=r, pend=pend, lvl=0, dir=d, m=m[:], e=[r]))
             assert top + len(pushes) <= FR
@@ -286,9 +344,12 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                                 rows.append(tuple(g["Euid"][x] for x in e))
                             cnt += 1
                         else:
-                            cnt += leaf_count(g, md, j + 1, m, pos, f["pend"], scan_max)
+                            L = cand_start(g, md, j + 1, m, pos)
+                            cnt += leaf_count(g, md, j + 1, m, L, pos, f["pend"], scan_max)
                         continue
-                    cb, ce, d = child_range(g, md, j + 1, m, pos, f["pend"])
+                    L = cand_start(g, md, j + 1, m, pos)
+                    cb, d = L[2], L[4]
+                    ce = gallop_lb(L[0], L[2], L[3], f["pend"])
                     if ce > cb:
                         children.append(dict(cur=cb, end=ce, p=pos, pend=f["pend"], lvl=j, dir=d, m=m, e=e))
                 assert ntop + len(children) <= FR, "frame stack overflow"

@@ -1,0 +1,45 @@
+"""The CPU model of the gfx950 kernel (tests/kernel_model.py) against the
+oracle: same counts and enumerated row sets, every index in bounds, frame stack
+within capacity, for all start rules, scan thresholds and chunk sizes."""
+import numpy as np
+
+import kernel_model as km
+from oracle import oracle, prep
+from helpers import random_graph, random_motif
+
+
+def test_model_matches_oracle():
+    rng = np.random.default_rng(7)
+    for _ in range(150):
+        E = int(rng.integers(1, 250))
+        U, V, T = random_graph(rng, E, int(rng.integers(2, 12)), int(rng.integers(1, 60)))
+        M = int(rng.integers(2, 7))
+        mot = random_motif(rng, M)
+        delta = int(rng.integers(0, 30))
+        g = prep.preprocess(U, V, T)
+        gv = km.build_view(g)
+        md = km.build_motif([(a, b) for a, b, _ in mot])
+        want = oracle.count(g, mot, delta, nthreads=1)
+        scan = int(rng.choice([0, 2, 16]))
+        got, _, st = km.run_wave(gv, md, delta, 0, g["E"] - M + 1, scan_max=scan,
+                                 chunk_roots=int(rng.choice([64, 128, 256])))
+        assert got == want
+        if want < 3000:
+            total, rows = oracle.enum(g, mot, delta, 100000)
+            c2, rows2, _ = km.run_wave(gv, md, delta, 0, g["E"] - M + 1, enumerate_=True)
+            assert c2 == total
+            assert sorted(rows2) == sorted(map(tuple, rows.tolist()))
+
+
+def test_model_root_ranges_are_additive():
+    rng = np.random.default_rng(8)
+    U, V, T = random_graph(rng, 400, 10, 100)
+    g = prep.preprocess(U, V, T)
+    gv = km.build_view(g)
+    for _ in range(10):
+        mot = random_motif(rng, 4)
+        md = km.build_motif([(a, b) for a, b, _ in mot])
+        cut = int(rng.integers(0, 397))
+        a, _, _ = km.run_wave(gv, md, 40, 0, cut)
+        b, _, _ = km.run_wave(gv, md, 40, cut, 397)
+        assert a + b == oracle.count(g, mot, 40)
