@@ -59,8 +59,10 @@ def lib():
                               ctypes.POINTER(I64)]
         L.tm_batch_count.argtypes = [P, I32, P, P, P, P, ctypes.POINTER(Tuning), P]
         L.tm_last_stats.argtypes = [P, P, I32]
+        L.tm_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+        L.tm_count_work.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P, P]
         for name in ("tm_graph_upload", "tm_graph_free", "tm_graph_info", "tm_count", "tm_enum",
-                     "tm_batch_count", "tm_last_stats"):
+                     "tm_batch_count", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -118,6 +120,21 @@ class DeviceGraph:
         st = np.zeros(8, dtype=np.uint64)
         check(lib().tm_last_stats(self.handle, _p(st), 8))
         return dict(steps=int(st[0]), items=int(st[1]), roots=int(st[2]), leaves=int(st[3]), pushes=int(st[4]))
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double()
+        check(lib().tm_last_kernel_ms(self.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def count_work(self, edges, delta, beg=0, end=-1, tune=None):
+        """(count, [R, X1, X2, P]) with the reference's work counted on the GPU."""
+        src, dst = motif_arrays(edges)
+        out = np.zeros(1, dtype=np.uint64)
+        work = np.zeros(4, dtype=np.uint64)
+        t = tune if tune is not None else Tuning()
+        check(lib().tm_count_work(self.handle, len(edges), _p(src), _p(dst), int(delta), int(beg), int(end),
+                                  ctypes.byref(t), _p(out), _p(work)))
+        return int(out[0]), [int(x) for x in work]
 
     def count(self, edges, delta, beg=0, end=-1, tune=None):
         src, dst = motif_arrays(edges)

@@ -131,6 +131,8 @@ struct tm_graph {
   std::vector<void *> allocs;
   int64_t bytes = 0;
   tm_graph_view view{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_kernel_ms = 0.f;
   tmh::Scratch scratch;
   unsigned long long last_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::mutex mtx;  // one launch at a time per graph handle
@@ -187,22 +189,41 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
   return TM_OK;
 }
 
-template <int MAXN, bool ENUM>
+template <int MAXN, bool ENUM, bool INSTR>
 static tm_status occupancy(int lds, int *blocks) {
-  HIPTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, tmk::mine_kernel<TM_MAXE, MAXN, ENUM>, 64, lds));
+  HIPTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR>, 64,
+                                                      lds));
   return TM_OK;
 }
 
-template <int MAXN, bool ENUM>
+template <int MAXN, bool ENUM, bool INSTR>
 static tm_status launch_t(tm_graph *g, const tm_kernel_args &a, int grid, int lds) {
-  hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM>), dim3(grid), dim3(64), lds, g->stream, a);
+  HIPTRY(hipEventRecord(g->ev0, g->stream));
+  hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR>), dim3(grid), dim3(64), lds, g->stream, a);
   HIPTRY(hipGetLastError());
+  HIPTRY(hipEventRecord(g->ev1, g->stream));
   return TM_OK;
+}
+
+// one of the five kernel variants: (nodes <= 4 | <= 8) x (count | enumerate),
+// plus the instrumented count
+template <bool OCC>
+static tm_status dispatch(tm_graph *g, int nmap, bool enumerate, bool instr, int lds, int *blocks,
+                          const tm_kernel_args *a, int grid) {
+  if (OCC) {
+    if (instr) return nmap == 4 ? occupancy<4, false, true>(lds, blocks) : occupancy<8, false, true>(lds, blocks);
+    if (nmap == 4) return enumerate ? occupancy<4, true, false>(lds, blocks) : occupancy<4, false, false>(lds, blocks);
+    return enumerate ? occupancy<8, true, false>(lds, blocks) : occupancy<8, false, false>(lds, blocks);
+  }
+  if (instr) return nmap == 4 ? launch_t<4, false, true>(g, *a, grid, lds) : launch_t<8, false, true>(g, *a, grid, lds);
+  if (nmap == 4)
+    return enumerate ? launch_t<4, true, false>(g, *a, grid, lds) : launch_t<4, false, false>(g, *a, grid, lds);
+  return enumerate ? launch_t<8, true, false>(g, *a, grid, lds) : launch_t<8, false, false>(g, *a, grid, lds);
 }
 
 // Plan and launch the mining kernel for queries already staged in scratch.
 tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const tm_tuning *tune,
-                 long long total_chunks, int chunk_roots) {
+                 long long total_chunks, int chunk_roots, bool instr) {
   Scratch &s = g->scratch;
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
@@ -210,9 +231,7 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   const int per_frame = 5 + nmap + (enumerate ? TM_MAXE : 0);
   const int lds = frames * per_frame * (int)sizeof(int);
   int blocks = 0;
-  tm_status st;
-  if (nmap == 4) st = enumerate ? occupancy<4, true>(lds, &blocks) : occupancy<4, false>(lds, &blocks);
-  else st = enumerate ? occupancy<8, true>(lds, &blocks) : occupancy<8, false>(lds, &blocks);
+  tm_status st = dispatch<true>(g, nmap, enumerate, instr, lds, &blocks, nullptr, 0);
   if (st != TM_OK) return st;
   int wpc = (tune && tune->waves_per_cu > 0) ? tune->waves_per_cu : 32;
   if (blocks > 0) wpc = std::min(wpc, blocks);
@@ -237,9 +256,7 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, g->stream, s.d_sched,
                      s.d_counts, s.d_stops, nq);
   HIPTRY(hipGetLastError());
-  if (nmap == 4) st = enumerate ? launch_t<4, true>(g, a, grid, lds) : launch_t<4, false>(g, a, grid, lds);
-  else st = enumerate ? launch_t<8, true>(g, a, grid, lds) : launch_t<8, false>(g, a, grid, lds);
-  return st;
+  return dispatch<false>(g, nmap, enumerate, instr, lds, &blocks, &a, grid);
 }
 
 // Reserve query staging space for nq queries (grows, never shrinks).
@@ -331,6 +348,10 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   }
   he = hipStreamCreate(&g->stream);
   if (he != hipSuccess) { delete g; return fail(TM_ERR_HIP, hipGetErrorString(he)); }
+  if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+    tm_graph_free(g);
+    return fail(TM_ERR_HIP, "hipEventCreate failed");
+  }
 
   auto cleanup = [&](tm_status s) {
     tm_graph_free(g);
@@ -439,6 +460,8 @@ tm_status tm_graph_free(tm_graph *g) {
   (void)hipSetDevice(g->device);
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   for (void *p : g->allocs) (void)hipFree(p);
+  if (g->ev0) (void)hipEventDestroy(g->ev0);
+  if (g->ev1) (void)hipEventDestroy(g->ev1);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
   return TM_OK;
@@ -447,6 +470,12 @@ tm_status tm_graph_free(tm_graph *g) {
 tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n) {
   if (!g || !stats || n < 0) return fail(TM_ERR_ARG, "bad argument");
   for (int i = 0; i < n && i < 8; i++) stats[i] = g->last_stats[i];
+  return TM_OK;
+}
+
+tm_status tm_last_kernel_ms(const tm_graph *g, double *ms) {
+  if (!g || !ms) return fail(TM_ERR_ARG, "bad argument");
+  *ms = g->last_kernel_ms;
   return TM_OK;
 }
 
@@ -519,6 +548,8 @@ static tm_status finish(tm_graph *g, int nq, uint64_t *counts) {
   HIPTRY(hipMemcpyAsync(counts, s.d_counts, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, g->stream));
   HIPTRY(hipStreamSynchronize(g->stream));
   memcpy(g->last_stats, hs.stats, sizeof(g->last_stats));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) g->last_kernel_ms = ms;
   if (hs.error & 4) return fail(TM_ERR_TIMEOUT, "query exceeded its time limit; counts discarded");
   if (hs.error) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(hs.error) + ")");
   return TM_OK;
@@ -536,8 +567,30 @@ tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const in
                                &maxN);
   if (st != TM_OK) return st;
   if ((st = upload_staged(g, 1)) != TM_OK) return st;
-  if ((st = launch(g, 1, maxM, maxN, false, tune, chunks, cr)) != TM_OK) return st;
+  if ((st = launch(g, 1, maxM, maxN, false, tune, chunks, cr, false)) != TM_OK) return st;
   return finish(g, 1, count);
+}
+
+tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                        int32_t beg, int32_t end, const tm_tuning *tune, uint64_t *count, uint64_t *work) {
+  if (!g || !msrc || !mdst || !count || !work) return fail(TM_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(g->mtx);
+  HIPTRY(hipSetDevice(g->device));
+  const int cr = chunk_roots_of(tune);
+  long long chunks;
+  int maxM, maxN;
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, cr, &chunks, &maxM,
+                               &maxN);
+  if (st != TM_OK) return st;
+  if ((st = upload_staged(g, 1)) != TM_OK) return st;
+  if ((st = launch(g, 1, maxM, maxN, false, tune, chunks, cr, true)) != TM_OK) return st;
+  if ((st = finish(g, 1, count)) != TM_OK) return st;
+  const tm_query_dev &q = g->scratch.h_queries[0];
+  work[0] = (uint64_t)(q.root_end - q.root_beg);
+  work[1] = g->last_stats[TM_STAT_REF_X1];
+  work[2] = g->last_stats[TM_STAT_REF_X2];
+  work[3] = g->last_stats[TM_STAT_REF_P];
+  return TM_OK;
 }
 
 tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, const int32_t *msrc,
@@ -552,7 +605,7 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
                                &chunks, &maxM, &maxN);
   if (st != TM_OK) return st;
   if ((st = upload_staged(g, nq)) != TM_OK) return st;
-  if ((st = launch(g, nq, maxM, maxN, false, tune, chunks, cr)) != TM_OK) return st;
+  if ((st = launch(g, nq, maxM, maxN, false, tune, chunks, cr, false)) != TM_OK) return st;
   return finish(g, nq, counts);
 }
 
@@ -575,7 +628,7 @@ tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int
   s.h_queries[0].rows = d_rows;
   s.h_queries[0].cap = cap;
   if ((st = upload_staged(g, 1)) != TM_OK) { (void)hipFree(d_rows); return st; }
-  if ((st = launch(g, 1, maxM, maxN, true, tune, chunks, cr)) != TM_OK) { (void)hipFree(d_rows); return st; }
+  if ((st = launch(g, 1, maxM, maxN, true, tune, chunks, cr, false)) != TM_OK) { (void)hipFree(d_rows); return st; }
   st = finish(g, 1, count);
   if (st == TM_OK) {
     int64_t n = std::min<int64_t>((int64_t)*count, cap);

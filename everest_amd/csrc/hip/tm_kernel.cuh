@@ -246,7 +246,30 @@ __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g,
 // LDS frame stack, structure of arrays, capacity FR frames:
 //   cur, end, p (last matched position), pend, lvl (level | dir << 8),
 //   m[MAXN][FR] mapping, and in enumeration mode e[MAXE][FR] matched positions.
-template <int MAXE, int MAXN, bool ENUM>
+// Reference work of one resolved candidate list (the instrumented variant):
+// the reference scans list entries after p until the first one outside the
+// window, and that terminating probe counts too.  For a closing edge it scans
+// the shorter of in(m[b]) and out(m[a]) (ties to out).
+template <int MAXN>
+__device__ __forceinline__ unsigned long long ref_probes(const tm_graph_view &g, const tm_motif_dev &md, int k,
+                                                         const int (&m)[MAXN], int p, int pend) {
+  const int kind = md.kind[k];
+  const int *pos;
+  int lo, hi;
+  if (kind == TM_K_NEW_DST || (kind == TM_K_CLOSE && !(g.inR[sel(m, md.b[k]) + 1] - g.inR[sel(m, md.b[k])] <
+                                                        g.outR[sel(m, md.a[k]) + 1] - g.outR[sel(m, md.a[k])]))) {
+    const int x = sel(m, md.a[k]);
+    pos = g.outPos; lo = g.outR[x]; hi = g.outR[x + 1];
+  } else {
+    const int y = sel(m, md.b[k]);
+    pos = g.inPos; lo = g.inR[y]; hi = g.inR[y + 1];
+  }
+  lo = upper_bound_i(pos, lo, hi, p);
+  const int e = gallop_lb(pos, lo, hi, pend);
+  return (unsigned long long)(e - lo) + (e < hi ? 1ull : 0ull);
+}
+
+template <int MAXE, int MAXN, bool ENUM, bool INSTR>
 __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   extern __shared__ int lds[];
   const int lane = threadIdx.x;
@@ -265,6 +288,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
 
   unsigned long long cnt = 0;
   unsigned long long st_steps = 0, st_items = 0, st_roots = 0, st_leaves = 0, st_push = 0;
+  unsigned long long rx1 = 0, rx2 = 0, rp = 0;  // reference work (INSTR only)
   unsigned iter = 0;
   int qcur = -1;
   int top = 0;
@@ -339,9 +363,16 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         ok = L.lo < L.hi && (long long)g.T[L.pos[L.lo]] <= tl;
       }
       int pend = 0;
+      if constexpr (INSTR) {
+        if (r < root_end && u != v) {
+          const int pe = pend_of(g, tl);
+          (md.kind[1] == TM_K_CLOSE ? rx2 : rx1) += 1;
+          rp += ref_probes(g, md, 1, m, r, pe);
+        }
+      }
       if (ok) {
         pend = pend_of(g, tl);
-        ok = (pend - r - 1) >= (M - 1);
+        if (!INSTR) ok = (pend - r - 1) >= (M - 1);
       }
       const int last = ENUM ? M - 1 : M - 2;  // last motif edge handled per candidate
       if (last == 0) {
@@ -451,6 +482,12 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       }
     }
     const int last = ENUM ? M - 1 : M - 2;
+    if constexpr (INSTR) {
+      if (valid && j + 1 < M) {
+        (md.kind[j + 1] == TM_K_CLOSE ? rx2 : rx1) += 1;
+        rp += ref_probes(g, md, j + 1, m, pos, pend);
+      }
+    }
     if (j == last) {
       if constexpr (ENUM) {
         if (valid) {
@@ -516,6 +553,14 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     atomicAdd(&sched->stats[TM_STAT_ROOTS], w1);
     atomicAdd(&sched->stats[TM_STAT_LEAVES], w2);
     atomicAdd(&sched->stats[TM_STAT_PUSHES], w3);
+  }
+  if constexpr (INSTR) {
+    unsigned long long a = wave_sum64(rx1), b = wave_sum64(rx2), c = wave_sum64(rp);
+    if (lane == 0) {
+      atomicAdd(&sched->stats[TM_STAT_REF_X1], a);
+      atomicAdd(&sched->stats[TM_STAT_REF_X2], b);
+      atomicAdd(&sched->stats[TM_STAT_REF_P], c);
+    }
   }
 }
 

@@ -76,7 +76,16 @@ typedef struct {
   unsigned long long stats[8];  // work counters (see TM_STAT_*)
 } tm_sched;
 
-enum { TM_STAT_STEPS = 0, TM_STAT_ITEMS = 1, TM_STAT_ROOTS = 2, TM_STAT_LEAVES = 3, TM_STAT_PUSHES = 4 };
+enum {
+  TM_STAT_STEPS = 0,   // wavefront expansion steps
+  TM_STAT_ITEMS = 1,   // candidate slots examined
+  TM_STAT_ROOTS = 2,   // roots dispatched
+  TM_STAT_LEAVES = 3,  // last-edge range computations
+  TM_STAT_PUSHES = 4,  // frames pushed
+  TM_STAT_REF_X1 = 5,  // reference list resolutions, new-node edges (instrumented run)
+  TM_STAT_REF_X2 = 6,  // reference list resolutions, closing edges (instrumented run)
+  TM_STAT_REF_P = 7    // reference candidate probes (instrumented run)
+};
 
 typedef struct {
   tm_graph_view g;

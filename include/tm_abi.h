@@ -111,6 +111,24 @@ tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const in
                    uint64_t *count);
 
 /*
+ * tm_count plus the reference's own work for the same query, counted exactly
+ * by an instrumented kernel: work[0] = R roots examined (end - beg),
+ * work[1] = X1 candidate-list resolutions of new-node edges, work[2] = X2
+ * resolutions of closing edges, work[3] = P candidate probes, including the
+ * probe that ends each scan at the window (the reference's scan loop,
+ * system/templates/common/nextMatchInTask.snp:1-21, resolveMi.snp:1-29).
+ * Algorithmic bytes of the reference: 12 R + 8 X1 + 16 X2 + 16 P.
+ * No reference counterpart (measurement).
+ */
+tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                        int32_t delta, int32_t beg, int32_t end, const tm_tuning *tune,
+                        uint64_t *count, uint64_t *work);
+
+/* Duration in ms of the graph's last mining kernel, from HIP events recorded
+ * around it on the library's stream.  No reference counterpart (measurement). */
+tm_status tm_last_kernel_ms(const tm_graph *g, double *ms);
+
+/*
  * Enumerate matches: writes up to `cap` rows of num_edges int32 original line
  * numbers (Euid of p_0 .. p_{M-1}) to `rows` (host, cap * num_edges ints) and
  * the number of matches found to *count.  When *count <= cap every match is

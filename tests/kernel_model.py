@@ -236,9 +236,24 @@ def leaf_count(g, md, j, m, L, p, pend, scan_max):
     return c
 
 
-def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=256, cap=1 << 30):
+def ref_probes(g, md, k, m, p, pend):
+    """Reference probes of one resolved list (instrumented kernel variant)."""
+    kind = md["kind"][k]
+    x, y = m[md["a"][k]], m[md["b"][k]]
+    use_out = kind == K_NEW_DST or (kind == K_CLOSE and not (
+        g["inR"][y + 1] - g["inR"][y] < g["outR"][x + 1] - g["outR"][x]))
+    if use_out:
+        pos, lo, hi = g["outPos"], g["outR"][x], g["outR"][x + 1]
+    else:
+        pos, lo, hi = g["inPos"], g["inR"][y], g["inR"][y + 1]
+    lo = upper_bound(pos, lo, hi, p)
+    e = gallop_lb(pos, lo, hi, pend)
+    return (e - lo) + (1 if e < hi else 0)
+
+
+def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=256, cap=1 << 30, instr=False):
     """Simulate one wavefront mining roots [beg, end) alone; returns
-    (count, rows, stats)."""
+    (count, rows, stats).  With instr, stats["ref"] = [R, X1, X2, P]."""
     M = md["M"]
     MAXN = 8
     lvls = max(1, (M - 1) if enumerate_ else (M - 2))
@@ -248,6 +263,8 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
     cnt = 0
     rows = []
     stats = dict(steps=0, items=0, max_top=0)
+    ref = [max(0, end - beg), 0, 0, 0]
+    stats["ref"] = ref
     last = (M - 1) if enumerate_ else (M - 2)
     Eg = g["Eg"]
     nchunks = (end - beg + chunk_roots - 1) // chunk_roots
@@ -268,10 +285,13 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                 m = [0] * MAXN
                 m[0], m[1] = u, v
                 L = cand_start(g, md, 1, m, r)
+                if instr:
+                    ref[2 if md["kind"][1] == K_CLOSE else 1] += 1
+                    ref[3] += ref_probes(g, md, 1, m, r, pend_of(g, t + delta))
                 if not (L[2] < L[3] and g["T"][L[0][L[2]]] <= t + delta):
                     continue
                 pend = pend_of(g, t + delta)
-                if pend - r - 1 < M - 1:
+                if pend - r - 1 < M - 1 and not instr:
                     continue
                 if last == 0:
                     cnt += leaf_count(g, md, 1, m, L, r, pend, scan_max)
@@ -337,6 +357,9 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                         valid = True
                     if not valid:
                         continue
+                    if instr and j + 1 < M:
+                        ref[2 if md["kind"][j + 1] == K_CLOSE else 1] += 1
+                        ref[3] += ref_probes(g, md, j + 1, m, pos, f["pend"])
                     e = f["e"] + [pos]
                     if j == last:
                         if enumerate_:

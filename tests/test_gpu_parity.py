@@ -91,3 +91,42 @@ def test_zipf100k_all36():
         got = dg.count(mot, 3600)
         assert got == want, (mot, got, want)
     dg.close()
+
+
+def test_reference_work_counters(small_cases):
+    """The instrumented kernel's R/X1/X2/P equal the oracle's counters."""
+    abi = _abi()
+    for (i, U, V, T, mot, delta) in small_cases[:40]:
+        g = prep.preprocess(U, V, T)
+        say("work case", i, "motif", mot, "delta", delta)
+        want_c, want_w = oracle.count(g, mot, delta, nthreads=2, with_work=True)
+        dg = abi.DeviceGraph.from_dict(g)
+        got_c, got_w = dg.count_work(mot, delta)
+        dg.close()
+        assert got_c == want_c and got_w == want_w, (i, got_w, want_w)
+
+
+def test_pl8m_root_subranges_and_timeouts():
+    """Bit-exact on the 8M-edge power-law graph over seeded root subranges
+    (counts are additive over root ranges), all 36 3-edge motifs."""
+    abi = _abi()
+    from everest_amd import core
+    from everest_amd.gen import graphs, queries
+    U, V, T = graphs.make("pl8m")
+    el = core.data.EdgeListLoader.fromArrays("pl8m", U, V, T)
+    hg = core.data.GraphDataLoader(el, core.data.FeatureLoader(el, "")).createGraphData()
+    arrs = {k: hg.array(k) for k in ("Eg", "inEdgesV", "outEdgesV", "inEdgesR", "outEdgesR", "Euid")}
+    g = dict(E=hg.numEdges, Eg=arrs["Eg"].reshape(-1, 3), inV=arrs["inEdgesV"], outV=arrs["outEdgesV"],
+             inR=arrs["inEdgesR"], outR=arrs["outEdgesR"], Euid=arrs["Euid"])
+    dg = abi.DeviceGraph(arrs["Eg"], arrs["inEdgesV"], arrs["outEdgesV"], arrs["inEdgesR"], arrs["outEdgesR"],
+                         arrs["Euid"])
+    rng = np.random.default_rng(21)
+    for edges in queries.all_3edge_motifs():
+        mot = queries.as_motif(edges)
+        beg = int(rng.integers(0, hg.numEdges - 200_000))
+        end = beg + 100_000
+        say("pl8m motif", edges, "roots", beg, end)
+        want = oracle.count(g, mot, 86400, beg, end, nthreads=8)
+        got = dg.count(mot, 86400, beg, end)
+        assert got == want, (edges, got, want)
+    dg.close()

@@ -43,3 +43,20 @@ def test_model_root_ranges_are_additive():
         a, _, _ = km.run_wave(gv, md, 40, 0, cut)
         b, _, _ = km.run_wave(gv, md, 40, cut, 397)
         assert a + b == oracle.count(g, mot, 40)
+
+
+def test_model_reference_work_counters_match_oracle():
+    """The instrumented variant's R/X1/X2/P equal the C oracle's counters."""
+    rng = np.random.default_rng(12)
+    for _ in range(80):
+        U, V, T = random_graph(rng, int(rng.integers(1, 200)), int(rng.integers(2, 10)), int(rng.integers(1, 40)))
+        M = int(rng.integers(2, 6))
+        mot = random_motif(rng, M)
+        delta = int(rng.integers(0, 25))
+        g = prep.preprocess(U, V, T)
+        gv = km.build_view(g)
+        md = km.build_motif([(a, b) for a, b, _ in mot])
+        want_c, want_w = oracle.count(g, mot, delta, nthreads=1, with_work=True)
+        got, _, st = km.run_wave(gv, md, delta, 0, g["E"] - M + 1, instr=True)
+        assert got == want_c
+        assert st["ref"] == want_w
