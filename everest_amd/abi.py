@@ -34,7 +34,8 @@ class HostGraph(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_roots", ctypes.c_int32),
                 ("scan_max", ctypes.c_int32), ("time_limit_ms", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("steal_every", ctypes.c_int32), ("steal_max", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 _lib = None
@@ -77,10 +78,10 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def tuning(waves_per_cu=0, chunk_roots=0, scan_max=0, time_limit_ms=0):
+def tuning(waves_per_cu=0, chunk_roots=0, scan_max=0, time_limit_ms=0, steal_every=0, steal_max=0):
     t = Tuning()
     t.waves_per_cu, t.chunk_roots, t.scan_max = waves_per_cu, chunk_roots, scan_max
-    t.time_limit_ms = time_limit_ms
+    t.time_limit_ms, t.steal_every, t.steal_max = time_limit_ms, steal_every, steal_max
     return t
 
 

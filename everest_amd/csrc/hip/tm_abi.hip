@@ -82,6 +82,38 @@ __global__ void next_kernel(const tm_edge *__restrict__ Eg, int E, const int *__
   inNext[p] = lo;
 }
 
+// Time of the first candidate after p for each start rule (INT_MAX: none).
+__global__ void tfirst_kernel(const tm_edge *__restrict__ Eg, const int *__restrict__ T, int E,
+                              const int *__restrict__ outR, const int *__restrict__ outPos,
+                              const int *__restrict__ inR, const int *__restrict__ inPos,
+                              const int *__restrict__ outSelf, const int *__restrict__ inSelf,
+                              const int *__restrict__ outNext, const int *__restrict__ inNext,
+                              const int *__restrict__ outPairNbr, const int *__restrict__ outPairPos,
+                              const int *__restrict__ outPairSelf, int *t0, int *t1, int *t2, int *t3, int *t4,
+                              int *t5) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  const tm_edge e = Eg[p];
+  const int NONE = 0x7fffffff;
+  int s = outSelf[p] + 1;
+  t0[p] = s < outR[e.u + 1] ? T[outPos[s]] : NONE;
+  s = outNext[p];
+  t1[p] = s < outR[e.v + 1] ? T[outPos[s]] : NONE;
+  s = inSelf[p] + 1;
+  t2[p] = s < inR[e.v + 1] ? T[inPos[s]] : NONE;
+  s = inNext[p];
+  t3[p] = s < inR[e.u + 1] ? T[inPos[s]] : NONE;
+  s = outPairSelf[p] + 1;
+  t4[p] = (s < outR[e.u + 1] && outPairNbr[s] == e.v) ? T[outPairPos[s]] : NONE;
+  // reciprocal: first edge v -> u after p, in the out-pair list of v
+  int lo = outR[e.v], hi = outR[e.v + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (outPairNbr[mid] < e.u || (outPairNbr[mid] == e.u && outPairPos[mid] <= p)) lo = mid + 1; else hi = mid;
+  }
+  t5[p] = (lo < outR[e.v + 1] && outPairNbr[lo] == e.u) ? T[outPairPos[lo]] : NONE;
+}
+
 __global__ void low_word_kernel(const unsigned long long *__restrict__ k, int *__restrict__ out, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (int)(unsigned)(k[i] & 0xffffffffull);
@@ -106,6 +138,10 @@ __global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops
     s->next_chunk = 0;
     s->error = 0;
     s->abort = 0;
+    s->pool_head = 0;
+    s->pool_tail = 0;
+    s->busy = 0;
+    s->idle = 0;
     s->t0 = (long long)wall_clock64();
   }
   if (i < 8) s->stats[i] = 0;
@@ -157,6 +193,7 @@ tm_status dev_alloc(tm_graph *g, void **p, size_t bytes) {
 tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_dev *md) {
   if (M < 2 || M > TM_MAXE) return fail(TM_ERR_MOTIF, "motif must have 2..8 edges");
   memset(md, 0, sizeof(*md));
+  for (int j = 0; j < TM_MAXE; j++) md->tmode[j] = TM_T_NONE;
   md->M = M;
   if (src[0] != 0 || dst[0] != 1)
     return fail(TM_ERR_MOTIF, "motif nodes must be renamed by first appearance (edge 0 = 0->1)");
@@ -172,13 +209,16 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
     if (a < n && b < n) {
       md->kind[j] = TM_K_CLOSE;
       md->smode[j] = (a == pa && b == pb) ? TM_S_PAIR_SELF : TM_S_SEARCH;
+      md->tmode[j] = (a == pa && b == pb) ? TM_T_PAIR_SELF : ((a == pb && b == pa) ? TM_T_RECIP : TM_T_NONE);
     } else if (a < n && b == n) {
       md->kind[j] = TM_K_NEW_DST;
       md->smode[j] = a == pa ? TM_S_OUT_SELF : (a == pb ? TM_S_OUT_NEXT : TM_S_SEARCH);
+      md->tmode[j] = a == pa ? TM_T_OUT_SELF : (a == pb ? TM_T_OUT_NEXT : TM_T_NONE);
       n++;
     } else if (b < n && a == n) {
       md->kind[j] = TM_K_NEW_SRC;
       md->smode[j] = b == pb ? TM_S_IN_SELF : (b == pa ? TM_S_IN_NEXT : TM_S_SEARCH);
+      md->tmode[j] = b == pb ? TM_T_IN_SELF : (b == pa ? TM_T_IN_NEXT : TM_T_NONE);
       n++;
     } else {
       return fail(TM_ERR_MOTIF, "motif is not prefix-connected with first-appearance node ids");
@@ -228,7 +268,7 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
   const int nmap = maxN <= 4 ? 4 : 8;
-  const int per_frame = 5 + nmap + (enumerate ? TM_MAXE : 0);
+  const int per_frame = 6 + nmap + (enumerate ? TM_MAXE : 0);
   const int lds = frames * per_frame * (int)sizeof(int);
   int blocks = 0;
   tm_status st = dispatch<true>(g, nmap, enumerate, instr, lds, &blocks, nullptr, 0);
@@ -252,6 +292,12 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   a.limit_ticks = 0;
   if (tune && tune->time_limit_ms > 0) a.limit_ticks = (long long)tune->time_limit_ms * (g->wall_rate_khz);
   a.sched = s.d_sched;
+  a.pool = s.d_pool;
+  a.pool_ready = s.d_pool_ready;
+  a.pool_cap = s.pool_cap;
+  a.gen = ++s.gen;
+  a.steal_every = (tune && tune->steal_every > 0) ? tune->steal_every : 16;
+  a.steal_max = (tune && tune->steal_max > 0) ? tune->steal_max : 4;
 
   hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, g->stream, s.d_sched,
                      s.d_counts, s.d_stops, nq);
@@ -277,6 +323,13 @@ tm_status reserve(tm_graph *g, int nq) {
   if (!s.d_sched) {
     if ((st = dev_alloc(g, &p, sizeof(tm_sched))) != TM_OK) return st;
     s.d_sched = (tm_sched *)p;
+    s.pool_cap = 1 << 20;
+    if ((st = dev_alloc(g, &p, sizeof(tm_pool_frame) * (size_t)s.pool_cap)) != TM_OK) return st;
+    s.d_pool = (tm_pool_frame *)p;
+    if ((st = dev_alloc(g, &p, sizeof(int) * (size_t)s.pool_cap)) != TM_OK) return st;
+    s.d_pool_ready = (int *)p;
+    HIPTRY(hipMemsetAsync(s.d_pool_ready, 0, sizeof(int) * (size_t)s.pool_cap, g->stream));
+    HIPTRY(hipStreamSynchronize(g->stream));
   }
   s.cap_q = cap;
   s.h_queries.resize(cap);
@@ -384,6 +437,9 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   ALLOC(vw.outNext, e4);
   ALLOC(vw.inNext, e4);
   ALLOC(vw.outPairSelf, e4);
+  for (int k = 0; k < TM_NTFIRST; k++) {
+    ALLOC(vw.tfirst[k], e4);
+  }
   const int tmin = Eg[0].t, tmax = Eg[E - 1].t;
   const long long span = (long long)tmax - tmin + 1;
   int bshift = ceil_log2(std::max(1ll, (span * 16) / std::max(1, E)));
@@ -442,6 +498,11 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
     hipLaunchKernelGGL(self_kernel, gr, bl, 0, s, vw.outPairPos, E, (int *)vw.outPairSelf);
     hipLaunchKernelGGL(next_kernel, gr, bl, 0, s, vw.Eg, E, vw.outR, vw.outPos, vw.inR, vw.inPos,
                        (int *)vw.outNext, (int *)vw.inNext);
+    TMPCHK(hipGetLastError());
+    hipLaunchKernelGGL(tfirst_kernel, gr, bl, 0, s, vw.Eg, vw.T, E, vw.outR, vw.outPos, vw.inR, vw.inPos,
+                       vw.outSelf, vw.inSelf, vw.outNext, vw.inNext, vw.outPairNbr, vw.outPairPos,
+                       vw.outPairSelf, (int *)vw.tfirst[0], (int *)vw.tfirst[1], (int *)vw.tfirst[2],
+                       (int *)vw.tfirst[3], (int *)vw.tfirst[4], (int *)vw.tfirst[5]);
     TMPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((nb + 1 + 255) / 256)), dim3(256), 0, s, vw.T, E, tmin,

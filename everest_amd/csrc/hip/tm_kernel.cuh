@@ -99,6 +99,20 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// device-scope (all XCDs) atomics for the donation protocol
+__device__ __forceinline__ int ld_acq(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_acq(const unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int add_ar(int *p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // first position whose time is > tl (the window end as a position bound)
 __device__ __forceinline__ int pend_of(const tm_graph_view &g, long long tl) {
   long long k = (tl - (long long)g.tmin) >> g.bshift;
@@ -279,7 +293,8 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   int *f_p = f_end + FR;
   int *f_pend = f_p + FR;
   int *f_lvl = f_pend + FR;
-  int *f_m = f_lvl + FR;
+  int *f_tl = f_lvl + FR;
+  int *f_m = f_tl + FR;
   int *f_e = f_m + MAXN * FR;
 
   const tm_graph_view g = args.g;
@@ -295,6 +310,22 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   int chunk_left = 0;      // 64-root groups left in the current chunk
   int next_root = 0;       // first root of the next 64-root group (uniform)
   int root_end = 0;
+  bool roots_done = false; // the root chunks of every query are handed out
+  bool is_idle = false;    // counted in sched->idle (else in sched->busy)
+  if (lane == 0) add_ar(&sched->busy, 1);
+
+  // copy one frame between the LDS stack (slot w) and a pool record, one field per lane
+  auto frame_field = [&](int f, int w) -> int * {
+    if (f == 1) return &f_lvl[w];
+    if (f == 2) return &f_cur[w];
+    if (f == 3) return &f_end[w];
+    if (f == 4) return &f_p[w];
+    if (f == 5) return &f_pend[w];
+    if (f == 6) return &f_tl[w];
+    if (f >= 7 && f < 7 + MAXN) return &f_m[(f - 7) * FR + w];
+    if (ENUM && f >= 7 + TM_MAXN && f < 7 + TM_MAXN + MAXE) return &f_e[(f - 7 - TM_MAXN) * FR + w];
+    return nullptr;
+  };
 
   for (;;) {
     if ((++iter & 63u) == 0u) {  // deadline check (wall clock, uniform)
@@ -305,13 +336,79 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         break;
       }
     }
+    if (top == 0 && roots_done) {
+      // ------------------------------------------ idle: take donated work
+      if (!is_idle) {
+        if (lane == 0) { add_ar(&sched->busy, -1); add_ar(&sched->idle, 1); }
+        is_idle = true;
+      }
+      int slot = -1;
+      for (int spin = 0; spin < (1 << 22); spin++) {
+        int h = 0, t = 0;
+        if (lane == 0) { h = (int)ld_acq(&sched->pool_head); t = (int)ld_acq(&sched->pool_tail); }
+        h = uni(__shfl(h, 0));
+        t = uni(__shfl(t, 0));
+        if (h < t && h < args.pool_cap) {
+          int won = 0;
+          if (lane == 0) {
+            add_ar(&sched->busy, 1);
+            won = atomicCAS(&sched->pool_head, (unsigned)h, (unsigned)(h + 1)) == (unsigned)h;
+            if (!won) add_ar(&sched->busy, -1);
+          }
+          if (uni(__shfl(won, 0))) { slot = h; break; }
+          continue;
+        }
+        int b = 0, ab = 0;
+        if (lane == 0) { b = ld_acq(&sched->busy); ab = ld_acq(&sched->abort); }
+        if (uni(__shfl(ab, 0))) break;
+        if (uni(__shfl(b, 0)) == 0) {
+          // nobody holds work: anything reserved is already in the pool
+          if (lane == 0) { h = (int)ld_acq(&sched->pool_head); t = (int)ld_acq(&sched->pool_tail); }
+          h = uni(__shfl(h, 0));
+          t = uni(__shfl(t, 0));
+          if (h < t && h < args.pool_cap) continue;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+      if (slot < 0) break;
+      if (lane == 0) add_ar(&sched->idle, -1);
+      is_idle = false;
+      int rdy = 0;
+      if (lane == 0) {
+        for (int w = 0; w < (1 << 24); w++) {
+          if (ld_acq(&args.pool_ready[slot]) == args.gen) { rdy = 1; break; }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      if (!uni(__shfl(rdy, 0))) { if (lane == 0) atomicOr(&sched->error, 8); break; }
+      __threadfence();
+      const int *rec = (const int *)&args.pool[slot];
+      const int fv = lane < TM_POOL_FIELDS ? rec[lane] : 0;
+      const int fq = uni(__shfl(fv, 0));
+      if (fq != qcur) {
+        if (!ENUM && qcur >= 0) {
+          unsigned long long sm = wave_sum64(cnt);
+          if (lane == 0 && sm) atomicAdd(args.queries[qcur].count, sm);
+        }
+        cnt = 0;
+        qcur = fq;
+      }
+      if (lane < TM_POOL_FIELDS) {
+        int *dst = frame_field(lane, 0);
+        if (dst) *dst = fv;
+      }
+      top = 1;
+      __syncthreads();
+      continue;
+    }
     if (top == 0) {
       // ---------------------------------------------------- root dispatch
       if (chunk_left == 0) {
         long long c = 0;
         if (lane == 0) c = (long long)atomicAdd(&sched->next_chunk, 1ull);
         c = __shfl(c, 0);
-        if (c >= args.total_chunks) break;
+        if (c >= args.total_chunks) { roots_done = true; continue; }
         // locate the query owning chunk c (uniform binary search)
         int lo = 0, hi = args.nq;
         while (hi - lo > 1) {
@@ -358,6 +455,8 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       // candidates of edge 1; most roots stop at the first candidate's time
       CandList L{g.outPos, g.outNbr, 0, 0, 0};
       const long long tl = (long long)t + q.delta;
+      const int tli = tl > 0x7fffffffll ? 0x7fffffff : (int)tl;
+      if (ok && md.tmode[1] >= 0) ok = g.tfirst[md.tmode[1]][r] <= tli;  // streamed, coalesced
       if (ok) {
         L = cand_start(g, md, 1, m, r);
         ok = L.lo < L.hi && (long long)g.T[L.pos[L.lo]] <= tl;
@@ -391,7 +490,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       if (top + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
       if (ok) {
         int w = top + __popcll(bm & lt_mask);
-        f_cur[w] = cb; f_end[w] = ce; f_p[w] = r; f_pend[w] = pend; f_lvl[w] = 0 | (dir << 8);
+        f_cur[w] = cb; f_end[w] = ce; f_p[w] = r; f_pend[w] = pend; f_lvl[w] = 0 | (dir << 8); f_tl[w] = tli;
 #pragma unroll
         for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
         if constexpr (ENUM) f_e[w] = r;
@@ -440,7 +539,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     const int myf = top - 1 - rank;
     st_steps++;
     st_items += has ? 1 : 0;
-    int slot = 0, p = 0, pend = 0, dir = 0;
+    int slot = 0, p = 0, pend = 0, dir = 0, tl = 0;
     int m[MAXN];
     int e[ENUM ? MAXE : 1];
 #pragma unroll
@@ -449,6 +548,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       slot = f_cur[myf] + off;
       p = f_p[myf];
       pend = f_pend[myf];
+      tl = f_tl[myf];
       dir = f_lvl[myf] >> 8;
 #pragma unroll
       for (int i = 0; i < MAXN; i++) m[i] = f_m[i * FR + myf];
@@ -502,7 +602,8 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
           if ((long long)w + 1 >= q.cap) *((volatile int *)q.stop) = 1;
         }
       } else {
-        if (valid) {
+        const int tm = md.tmode[j + 1];
+        if (valid && (tm < 0 || g.tfirst[tm][pos] <= tl)) {
           const CandList LL = cand_start(g, md, j + 1, m, pos);
           cnt += leaf_count(g, md, j + 1, m, LL, pos, pend, args.scan_max);
           st_leaves++;
@@ -513,6 +614,10 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       continue;
     }
     int cb = 0, ce = 0, cdir = 0;
+    {
+      const int tm = md.tmode[j + 1];
+      if (valid && tm >= 0) valid = g.tfirst[tm][pos] <= tl;
+    }
     if (valid) {
       const CandList LL = cand_start(g, md, j + 1, m, pos);
       cb = LL.lo;
@@ -525,7 +630,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     if (ntop + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
     if (valid) {
       int w = ntop + __popcll(bm & lt_mask);
-      f_cur[w] = cb; f_end[w] = ce; f_p[w] = pos; f_pend[w] = pend; f_lvl[w] = j | (cdir << 8);
+      f_cur[w] = cb; f_end[w] = ce; f_p[w] = pos; f_pend[w] = pend; f_lvl[w] = j | (cdir << 8); f_tl[w] = tl;
 #pragma unroll
       for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
       if constexpr (ENUM) {
@@ -538,11 +643,50 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     st_push += valid ? 1 : 0;
     top = ntop + n;
     __syncthreads();
+    // ---------------------------------------------- donate to idle wavefronts
+    if (top > 0 && (st_steps % (unsigned long long)args.steal_every) == 0) {
+      int idl = 0;
+      if (lane == 0) idl = ld_acq(&sched->idle);
+      idl = uni(__shfl(idl, 0));
+      for (int d = 0; d < min(idl, args.steal_max); d++) {
+        // split the shallowest frame with at least two candidates left
+        const bool can = lane < top && (f_end[lane] - f_cur[lane] >= 2);
+        const unsigned long long cm = __ballot(can);
+        if (cm == 0ull) break;
+        const int b = __ffsll((long long)cm) - 1;
+        int t = 0;
+        if (lane == 0) t = (int)atomicAdd(&sched->pool_tail, 1u);
+        t = uni(__shfl(t, 0));
+        if (t >= args.pool_cap) break;
+        const int cur0 = f_cur[b], end0 = f_end[b];
+        const int mid = cur0 + (end0 - cur0) / 2;
+        int *rec = (int *)&args.pool[t];
+        if (lane < TM_POOL_FIELDS) {
+          int v = 0;
+          if (lane == 0) v = qcur;
+          else if (lane == 2) v = mid;
+          else {
+            int *src = frame_field(lane, b);
+            v = src ? *src : 0;
+          }
+          rec[lane] = v;
+        }
+        __syncthreads();
+        if (lane == 0) f_end[b] = mid;
+        __threadfence();
+        if (lane == 0) st_rel(&args.pool_ready[t], args.gen);
+        __syncthreads();
+      }
+    }
   }
 
   if (!ENUM && qcur >= 0) {
     unsigned long long s = wave_sum64(cnt);
     if (lane == 0 && s) atomicAdd(args.queries[qcur].count, s);
+  }
+  if (lane == 0) {
+    if (is_idle) add_ar(&sched->idle, -1);
+    else add_ar(&sched->busy, -1);
   }
   // work counters: one atomic per counter per wavefront
   unsigned long long w0 = wave_sum64(st_items), w1 = wave_sum64(st_roots), w2 = wave_sum64(st_leaves),

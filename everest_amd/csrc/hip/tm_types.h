@@ -25,6 +25,20 @@ enum {
   TM_S_PAIR_SELF = 5   // same pair as edge j-1: outPairSelf[p] + 1
 };
 
+// Time of the first candidate of edge j after position p, per start rule
+// (INT_MAX when there is none): one load decides that a list has nothing in
+// the window.
+enum {
+  TM_T_NONE = -1,
+  TM_T_OUT_SELF = 0,  // next out-edge of src p
+  TM_T_OUT_NEXT = 1,  // next out-edge of dst p
+  TM_T_IN_SELF = 2,   // next in-edge of dst p
+  TM_T_IN_NEXT = 3,   // next in-edge of src p
+  TM_T_PAIR_SELF = 4, // next edge src p -> dst p
+  TM_T_RECIP = 5      // next edge dst p -> src p
+};
+#define TM_NTFIRST 6
+
 typedef struct { int u, v, t; } tm_edge;  // reference TemporalEdge (include/data.h:27-29)
 
 typedef struct {
@@ -35,6 +49,7 @@ typedef struct {
   int b[TM_MAXE];       // motif destination node of edge j
   int mapped[TM_MAXE];  // nodes mapped before edge j (a new node gets this index)
   int smode[TM_MAXE];   // TM_S_* start rule of edge j's candidate list
+  int tmode[TM_MAXE];   // TM_T_* first-candidate time array of edge j (or TM_T_NONE)
 } tm_motif_dev;
 
 // Device-resident graph: reference arrays plus the indexes built at upload.
@@ -51,6 +66,7 @@ typedef struct {
   const int *outNext;     // first slot of out(dst of p) with position > p
   const int *inNext;      // first slot of in(src of p) with position > p
   const int *outPairSelf; // slot of position p in the out-pair index of src of p
+  const int *tfirst[TM_NTFIRST];  // TM_T_* arrays, one entry per position
   const int *bucket;      // bucket[k] = first position with T >= tmin + (k << bshift)
   int tmin, bshift;
   long long nbuckets;
@@ -70,11 +86,24 @@ typedef struct {
 
 typedef struct {
   unsigned long long next_chunk;
-  int error;        // bit 0: frame stack overflow, bit 1: empty frame, bit 2: deadline passed
+  int error;        // bit 0: frame stack overflow, bit 1: empty frame, bit 2: deadline passed,
+                    // bit 3: a donated frame never became ready
   int abort;        // set once the deadline passed: every wavefront stops
   long long t0;     // wall clock at launch (set by the reset kernel)
   unsigned long long stats[8];  // work counters (see TM_STAT_*)
+  unsigned int pool_head;  // next donated frame to take
+  unsigned int pool_tail;  // donated frames reserved so far
+  int busy;         // wavefronts holding work (may still donate)
+  int idle;         // wavefronts waiting for donated work
 } tm_sched;
+
+// A frame donated by a busy wavefront to an idle one (global pool).
+typedef struct {
+  int q, lvl, cur, end, p, pend, tl;
+  int m[TM_MAXN];
+  int e[TM_MAXE];
+} tm_pool_frame;
+#define TM_POOL_FIELDS 23
 
 enum {
   TM_STAT_STEPS = 0,   // wavefront expansion steps
@@ -98,4 +127,10 @@ typedef struct {
   int scan_max;               // last-edge ranges up to this size are scanned
   long long limit_ticks;      // wall-clock budget of the launch (0 = none)
   tm_sched *sched;
+  tm_pool_frame *pool;        // donated frames
+  int *pool_ready;            // pool_ready[i] == gen once pool[i] is written
+  int pool_cap;
+  int gen;                    // launch generation tag for pool_ready
+  int steal_every;            // busy wavefronts look for idle ones every this many steps
+  int steal_max;              // frames donated per look
 } tm_kernel_args;

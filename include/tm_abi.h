@@ -64,7 +64,9 @@ typedef struct tm_tuning {
   int32_t chunk_roots;   /* roots per dispatched chunk, multiple of 64 */
   int32_t scan_max;      /* last-edge ranges up to this size are scanned, larger ones use the pair index */
   int32_t time_limit_ms; /* abort the launch after this wall time on the device (0 = no limit) */
-  int32_t reserved[4];
+  int32_t steal_every;   /* busy wavefronts look for idle ones every this many steps */
+  int32_t steal_max;     /* frames a busy wavefront donates per look */
+  int32_t reserved[2];
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */

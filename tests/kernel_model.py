@@ -61,7 +61,30 @@ def build_view(g):
                         for p in range(E)], dtype=np.int64)
     inNext = np.array([inR[Eg[p, 0]] + np.searchsorted(inPos[inR[Eg[p, 0]]:inR[Eg[p, 0] + 1]], p, side="right")
                        for p in range(E)], dtype=np.int64)
-    return dict(E=E, V=V, Eg=Eg, T=Arr(T, "T"),
+    NONE = 2**31 - 1
+    tf = np.full((6, E), NONE, dtype=np.int64)
+    outPairNbr = outNbr[o]
+    for p in range(E):
+        u_, v_ = int(Eg[p, 0]), int(Eg[p, 1])
+        s_ = outSelf[p] + 1
+        if s_ < outR[u_ + 1]:
+            tf[0, p] = T[outPos[s_]]
+        s_ = outNext[p]
+        if s_ < outR[v_ + 1]:
+            tf[1, p] = T[outPos[s_]]
+        s_ = inSelf[p] + 1
+        if s_ < inR[v_ + 1]:
+            tf[2, p] = T[inPos[s_]]
+        s_ = inNext[p]
+        if s_ < inR[u_ + 1]:
+            tf[3, p] = T[inPos[s_]]
+        s_ = outPairSelf[p] + 1
+        if s_ < outR[u_ + 1] and outPairNbr[s_] == v_:
+            tf[4, p] = T[outPairPos[s_]]
+        rec = [outPairPos[k] for k in range(outR[v_], outR[v_ + 1]) if outPairNbr[k] == u_ and outPairPos[k] > p]
+        if rec:
+            tf[5, p] = T[min(rec)]
+    return dict(E=E, V=V, Eg=Eg, T=Arr(T, "T"), tfirst=[Arr(tf[k], "tfirst%d" % k) for k in range(6)],
                 outSelf=Arr(outSelf, "outSelf"), inSelf=Arr(inSelf, "inSelf"),
                 outNext=Arr(outNext, "outNext"), inNext=Arr(inNext, "inNext"),
                 outPairSelf=Arr(outPairSelf, "outPairSelf"),
@@ -81,6 +104,7 @@ def build_motif(edges):
     M = len(edges)
     assert edges[0][0] == 0 and edges[0][1] == 1
     kind, A, B, mapped, smode = [0] * M, [0] * M, [0] * M, [0] * M, [0] * M
+    tmode = [-1] * M
     n = 2
     for j in range(1, M):
         a, b = edges[j][0], edges[j][1]
@@ -90,17 +114,20 @@ def build_motif(edges):
         if a < n and b < n:
             kind[j] = K_CLOSE
             smode[j] = S_PAIR_SELF if (a, b) == (pa, pb) else S_SEARCH
+            tmode[j] = 4 if (a, b) == (pa, pb) else (5 if (a, b) == (pb, pa) else -1)
         elif a < n and b == n:
             kind[j] = K_NEW_DST
             smode[j] = S_OUT_SELF if a == pa else (S_OUT_NEXT if a == pb else S_SEARCH)
+            tmode[j] = 0 if a == pa else (1 if a == pb else -1)
             n += 1
         elif b < n and a == n:
             kind[j] = K_NEW_SRC
             smode[j] = S_IN_SELF if b == pb else (S_IN_NEXT if b == pa else S_SEARCH)
+            tmode[j] = 2 if b == pb else (3 if b == pa else -1)
             n += 1
         else:
             raise ValueError("not prefix-connected")
-    return dict(M=M, n=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode)
+    return dict(M=M, n=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode)
 
 
 def lower_bound(a, lo, hi, x):
@@ -284,6 +311,8 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                     continue
                 m = [0] * MAXN
                 m[0], m[1] = u, v
+                if g["tfirst"][md["tmode"][1]][r] > t + delta and not instr:
+                    continue
                 L = cand_start(g, md, 1, m, r)
                 if instr:
                     ref[2 if md["kind"][1] == K_CLOSE else 1] += 1
@@ -299,7 +328,7 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                 cb, d = L[2], L[4]
                 ce = gallop_lb(L[0], L[2], L[3], pend)
                 if ce > cb:
-                    pushes.append(dict(cur=cb, end=ce, p=r, pend=pend, lvl=0, dir=d, m=m[:], e=[r]))
+                    pushes.append(dict(cur=cb, end=ce, p=r, pend=pend, lvl=0, dir=d, m=m[:], e=[r], tl=t + delta))
             assert top + len(pushes) <= FR
             for f in pushes:
                 frames[top] = f
@@ -361,22 +390,228 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                         ref[2 if md["kind"][j + 1] == K_CLOSE else 1] += 1
                         ref[3] += ref_probes(g, md, j + 1, m, pos, f["pend"])
                     e = f["e"] + [pos]
+                    tm = md["tmode"][j + 1] if j + 1 < M else -1
+                    empty_next = tm >= 0 and g["tfirst"][tm][pos] > f["tl"]
                     if j == last:
                         if enumerate_:
                             if cnt < cap:
                                 rows.append(tuple(g["Euid"][x] for x in e))
                             cnt += 1
-                        else:
+                        elif not empty_next:
                             L = cand_start(g, md, j + 1, m, pos)
                             cnt += leaf_count(g, md, j + 1, m, L, pos, f["pend"], scan_max)
+                        continue
+                    if empty_next:
                         continue
                     L = cand_start(g, md, j + 1, m, pos)
                     cb, d = L[2], L[4]
                     ce = gallop_lb(L[0], L[2], L[3], f["pend"])
                     if ce > cb:
-                        children.append(dict(cur=cb, end=ce, p=pos, pend=f["pend"], lvl=j, dir=d, m=m, e=e))
+                        children.append(dict(cur=cb, end=ce, p=pos, pend=f["pend"], lvl=j, dir=d, m=m, e=e, tl=f["tl"]))
                 assert ntop + len(children) <= FR, "frame stack overflow"
                 for k, ch in enumerate(children):
                     frames[ntop + k] = ch
                 top = ntop + len(children)
     return cnt, rows, stats
+
+
+# ------------------------------------------------------------ many wavefronts
+
+class _Shared:
+    def __init__(self, nchunks, pool_cap):
+        self.next_chunk = 0
+        self.nchunks = nchunks
+        self.pool = []          # donated frames, in reservation order
+        self.head = 0
+        self.cap = pool_cap
+        self.busy = 0
+        self.idle = 0
+
+
+class Wave:
+    """One wavefront of mine_kernel as a step machine (one loop iteration per
+    step), sharing the chunk counter and the donation pool with the others."""
+
+    def __init__(self, g, md, delta, beg, end, sh, scan_max, chunk_roots, steal_every, steal_max):
+        self.g, self.md, self.delta, self.beg, self.end = g, md, delta, beg, end
+        self.sh, self.scan_max, self.chunk_roots = sh, scan_max, chunk_roots
+        self.steal_every, self.steal_max = steal_every, steal_max
+        M = md["M"]
+        self.FR = WAVE * max(1, M - 2)
+        self.frames = [None] * self.FR
+        self.top = 0
+        self.cnt = 0
+        self.chunk_left = 0
+        self.next_root = 0
+        self.roots_done = False
+        self.is_idle = False
+        self.steps = 0
+        self.done = False
+        sh.busy += 1
+
+    def step(self):
+        g, md, sh = self.g, self.md, self.sh
+        M = md["M"]
+        last = M - 2
+        if self.top == 0 and self.roots_done:
+            if not self.is_idle:
+                sh.busy -= 1
+                sh.idle += 1
+                self.is_idle = True
+            if sh.head < len(sh.pool) and sh.head < sh.cap:
+                sh.busy += 1
+                f = sh.pool[sh.head]
+                sh.head += 1
+                sh.idle -= 1
+                self.is_idle = False
+                self.frames[0] = dict(f)
+                self.top = 1
+                return
+            if sh.busy == 0:
+                sh.idle -= 1
+                self.done = True
+            return  # spin
+        if self.top == 0:
+            if self.chunk_left == 0:
+                c = sh.next_chunk
+                sh.next_chunk += 1
+                if c >= sh.nchunks:
+                    self.roots_done = True
+                    return
+                first = self.beg + c * self.chunk_roots
+                self.next_root = first
+                self.chunk_left = min(self.chunk_roots // WAVE, (self.end - first + WAVE - 1) // WAVE)
+            pushes = []
+            for lane in range(WAVE):
+                r = self.next_root + lane
+                if r >= self.end:
+                    continue
+                u, v, t = (int(x) for x in g["Eg"][r])
+                if u == v:
+                    continue
+                m = [0] * 8
+                m[0], m[1] = u, v
+                if g["tfirst"][md["tmode"][1]][r] > t + self.delta:
+                    continue
+                L = cand_start(g, md, 1, m, r)
+                if not (L[2] < L[3] and g["T"][L[0][L[2]]] <= t + self.delta):
+                    continue
+                pend = pend_of(g, t + self.delta)
+                if pend - r - 1 < M - 1:
+                    continue
+                if last == 0:
+                    self.cnt += leaf_count(g, md, 1, m, L, r, pend, self.scan_max)
+                    continue
+                ce = gallop_lb(L[0], L[2], L[3], pend)
+                if ce > L[2]:
+                    pushes.append(dict(cur=L[2], end=ce, p=r, pend=pend, lvl=0, dir=L[4], m=m[:], tl=t + self.delta))
+            self.next_root += WAVE
+            self.chunk_left -= 1
+            assert self.top + len(pushes) <= self.FR
+            for f in pushes:
+                self.frames[self.top] = f
+                self.top += 1
+            return
+        # expansion step (same batching as run_wave)
+        top, frames = self.top, self.frames
+        L_ = frames[top - 1]["lvl"]
+        fcnt = [0] * WAVE
+        for lane in range(WAVE):
+            fi = top - 1 - lane
+            if fi >= 0 and frames[fi]["lvl"] == L_:
+                fcnt[lane] = frames[fi]["end"] - frames[fi]["cur"]
+        assert fcnt[0] > 0
+        incl = [int(x) for x in np.cumsum(fcnt)]
+        excl = [incl[i] - fcnt[i] for i in range(WAVE)]
+        total = min(WAVE, incl[-1])
+        part = [fcnt[i] > 0 and excl[i] < WAVE for i in range(WAVE)]
+        nparts = sum(part)
+        take = [min(fcnt[i], WAVE - excl[i]) if part[i] else 0 for i in range(WAVE)]
+        last_partial = any(part[i] and take[i] < fcnt[i] for i in range(WAVE))
+        sm = 0
+        for i in range(WAVE):
+            if part[i]:
+                sm |= 1 << excl[i]
+        items = []
+        for lane in range(total):
+            rank = bin(sm & ((2 << lane) - 1)).count("1") - 1
+            f = frames[top - 1 - rank]
+            slot = f["cur"] + lane - excl[rank]
+            assert f["cur"] <= slot < f["end"]
+            items.append((slot, f))
+        for i in range(WAVE):
+            if part[i] and take[i] < fcnt[i]:
+                frames[top - 1 - i] = dict(frames[top - 1 - i])
+                frames[top - 1 - i]["cur"] += take[i]
+        ntop = top - nparts + (1 if last_partial else 0)
+        j = L_ + 1
+        kind = md["kind"][j]
+        children = []
+        for slot, f in items:
+            m = list(f["m"])
+            if kind == K_NEW_DST:
+                pos, node = g["outPos"][slot], g["outNbr"][slot]
+                valid = node not in m[:md["mapped"][j]]
+                m[md["b"][j]] = node
+            elif kind == K_NEW_SRC:
+                pos, node = g["inPos"][slot], g["inNbr"][slot]
+                valid = node not in m[:md["mapped"][j]]
+                m[md["a"][j]] = node
+            else:
+                pos = g["inPairPos"][slot] if f["dir"] else g["outPairPos"][slot]
+                valid = True
+            if not valid:
+                continue
+            tm = md["tmode"][j + 1]
+            if tm >= 0 and g["tfirst"][tm][pos] > f["tl"]:
+                continue
+            if j == last:
+                Lc = cand_start(g, md, j + 1, m, pos)
+                self.cnt += leaf_count(g, md, j + 1, m, Lc, pos, f["pend"], self.scan_max)
+                continue
+            Lc = cand_start(g, md, j + 1, m, pos)
+            ce = gallop_lb(Lc[0], Lc[2], Lc[3], f["pend"])
+            if ce > Lc[2]:
+                children.append(dict(cur=Lc[2], end=ce, p=pos, pend=f["pend"], lvl=j, dir=Lc[4], m=m, tl=f["tl"]))
+        assert ntop + len(children) <= self.FR
+        for k, ch in enumerate(children):
+            frames[ntop + k] = ch
+        self.top = ntop + len(children)
+        self.steps += 1
+        # donation
+        if self.top > 0 and self.steps % self.steal_every == 0:
+            for _ in range(min(sh.idle, self.steal_max)):
+                cand = [i for i in range(min(self.top, WAVE)) if frames[i]["end"] - frames[i]["cur"] >= 2]
+                if not cand:
+                    break
+                b = cand[0]
+                t = len(sh.pool)
+                if t >= sh.cap:
+                    break
+                f = dict(frames[b])
+                mid = f["cur"] + (f["end"] - f["cur"]) // 2
+                don = dict(f)
+                don["cur"] = mid
+                sh.pool.append(don)
+                f["end"] = mid
+                frames[b] = f
+
+
+def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, steal_every=1, steal_max=4,
+              pool_cap=1 << 30, seed=0):
+    """Simulate nwaves wavefronts stepping in a random interleaving; returns
+    (count, number of donated frames)."""
+    rng = np.random.default_rng(seed)
+    nchunks = max(0, (end - beg + chunk_roots - 1) // chunk_roots)
+    sh = _Shared(nchunks, pool_cap)
+    waves = [Wave(g, md, delta, beg, end, sh, scan_max, chunk_roots, steal_every, steal_max)
+             for _ in range(nwaves)]
+    guard = 0
+    while not all(w.done for w in waves):
+        live = [w for w in waves if not w.done]
+        w = live[int(rng.integers(0, len(live)))]
+        w.step()
+        guard += 1
+        assert guard < 10_000_000, "no progress"
+    assert sh.head == min(len(sh.pool), sh.cap)
+    return sum(w.cnt for w in waves), len(sh.pool)

@@ -60,3 +60,25 @@ def test_model_reference_work_counters_match_oracle():
         got, _, st = km.run_wave(gv, md, delta, 0, g["E"] - M + 1, instr=True)
         assert got == want_c
         assert st["ref"] == want_w
+
+
+def test_model_work_donation_between_wavefronts():
+    """Several simulated wavefronts with frame splitting and a shared pool count
+    exactly what the oracle counts (no frame lost or processed twice)."""
+    rng = np.random.default_rng(31)
+    donated = 0
+    for _ in range(40):
+        U, V, T = random_graph(rng, int(rng.integers(50, 400)), int(rng.integers(2, 8)), int(rng.integers(1, 30)))
+        M = int(rng.integers(3, 6))
+        mot = random_motif(rng, M)
+        delta = int(rng.integers(5, 40))
+        g = prep.preprocess(U, V, T)
+        gv = km.build_view(g)
+        md = km.build_motif([(a, b) for a, b, _ in mot])
+        want = oracle.count(g, mot, delta, nthreads=1)
+        got, nd = km.run_waves(gv, md, delta, 0, g["E"] - M + 1, nwaves=int(rng.integers(2, 6)),
+                               steal_every=int(rng.integers(1, 4)), steal_max=int(rng.integers(1, 5)),
+                               pool_cap=int(rng.choice([2, 1 << 30])), seed=int(rng.integers(0, 1 << 30)))
+        assert got == want
+        donated += nd
+    assert donated > 0
