@@ -134,8 +134,8 @@ __global__ void bucket_kernel(const int *__restrict__ T, int E, int tmin, int bs
 
 __global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < TM_STRIPES) s->next_chunk[i][0] = 0;
   if (i == 0) {
-    s->next_chunk = 0;
     s->error = 0;
     s->abort = 0;
     s->pool_head = 0;
@@ -169,6 +169,7 @@ struct tm_graph {
   tm_graph_view view{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_kernel_ms = 0.f;
+  int last_chunk_roots = 0, last_grid = 0;
   tmh::Scratch scratch;
   unsigned long long last_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::mutex mtx;  // one launch at a time per graph handle
@@ -261,9 +262,10 @@ static tm_status dispatch(tm_graph *g, int nmap, bool enumerate, bool instr, int
   return enumerate ? launch_t<8, true, false>(g, *a, grid, lds) : launch_t<8, false, false>(g, *a, grid, lds);
 }
 
-// Plan and launch the mining kernel for queries already staged in scratch.
-tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const tm_tuning *tune,
-                 long long total_chunks, int chunk_roots, bool instr) {
+// Plan and launch the mining kernel for nq queries staged in scratch.h_queries:
+// size the persistent grid from occupancy, pick the chunk size, lay the
+// queries' root chunks out in one index space, upload, reset, launch.
+tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const tm_tuning *tune, bool instr) {
   Scratch &s = g->scratch;
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
@@ -277,7 +279,31 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   if (blocks > 0) wpc = std::min(wpc, blocks);
   if (wpc < 1) wpc = 1;
   int grid = g->num_cus * wpc;
+
+  long long roots = 0;
+  for (int q = 0; q < nq; q++) roots += (long long)(s.h_queries[q].root_end - s.h_queries[q].root_beg);
+  int chunk_roots;
+  if (tune && tune->chunk_roots > 0) {
+    chunk_roots = (tune->chunk_roots + 63) / 64 * 64;
+  } else {
+    // about 8 chunks per wavefront: few atomics, still fine-grained at the end
+    long long c = (roots + (long long)grid * 8 - 1) / ((long long)grid * 8);
+    c = (c + 63) / 64 * 64;
+    chunk_roots = (int)std::min<long long>(std::max<long long>(c, 256), 8192);
+  }
+  long long acc = 0;
+  for (int q = 0; q < nq; q++) {
+    s.h_prefix[q] = acc;
+    acc += ((long long)(s.h_queries[q].root_end - s.h_queries[q].root_beg) + chunk_roots - 1) / chunk_roots;
+  }
+  s.h_prefix[nq] = acc;
+  const long long total_chunks = acc;
   if (total_chunks < grid) grid = (int)std::max<long long>(1, total_chunks);
+
+  HIPTRY(hipMemcpyAsync(s.d_queries, s.h_queries.data(), sizeof(tm_query_dev) * nq, hipMemcpyHostToDevice,
+                        g->stream));
+  HIPTRY(hipMemcpyAsync(s.d_prefix, s.h_prefix.data(), sizeof(long long) * (nq + 1), hipMemcpyHostToDevice,
+                        g->stream));
 
   tm_kernel_args a;
   memset(&a, 0, sizeof(a));
@@ -293,11 +319,13 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   if (tune && tune->time_limit_ms > 0) a.limit_ticks = (long long)tune->time_limit_ms * (g->wall_rate_khz);
   a.sched = s.d_sched;
   a.pool = s.d_pool;
-  a.pool_ready = s.d_pool_ready;
   a.pool_cap = s.pool_cap;
   a.gen = ++s.gen;
   a.steal_every = (tune && tune->steal_every > 0) ? tune->steal_every : 16;
   a.steal_max = (tune && tune->steal_max > 0) ? tune->steal_max : 4;
+  a.idle_cap = std::max(64, g->num_cus * 2);
+  g->last_chunk_roots = chunk_roots;
+  g->last_grid = grid;
 
   hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, g->stream, s.d_sched,
                      s.d_counts, s.d_stops, nq);
@@ -323,12 +351,11 @@ tm_status reserve(tm_graph *g, int nq) {
   if (!s.d_sched) {
     if ((st = dev_alloc(g, &p, sizeof(tm_sched))) != TM_OK) return st;
     s.d_sched = (tm_sched *)p;
-    s.pool_cap = 1 << 20;
-    if ((st = dev_alloc(g, &p, sizeof(tm_pool_frame) * (size_t)s.pool_cap)) != TM_OK) return st;
-    s.d_pool = (tm_pool_frame *)p;
-    if ((st = dev_alloc(g, &p, sizeof(int) * (size_t)s.pool_cap)) != TM_OK) return st;
-    s.d_pool_ready = (int *)p;
-    HIPTRY(hipMemsetAsync(s.d_pool_ready, 0, sizeof(int) * (size_t)s.pool_cap, g->stream));
+    s.pool_cap = 1 << 18;
+    const size_t pool_bytes = sizeof(unsigned long long) * TM_POOL_FIELDS * (size_t)s.pool_cap;
+    if ((st = dev_alloc(g, &p, pool_bytes)) != TM_OK) return st;
+    s.d_pool = (unsigned long long *)p;
+    HIPTRY(hipMemsetAsync(s.d_pool, 0, pool_bytes, g->stream));  // generation 0 is never used
     HIPTRY(hipStreamSynchronize(g->stream));
   }
   s.cap_q = cap;
@@ -552,12 +579,10 @@ tm_status tm_graph_info(const tm_graph *g, int64_t *device_bytes, int32_t *num_e
 
 static tm_status stage_queries(tm_graph *g, int nq, const int32_t *num_edges, const int32_t *msrc,
                                const int32_t *mdst, int stride, const int32_t *deltas,
-                               const int32_t *begs, const int32_t *ends, int chunk_roots,
-                               long long *total_chunks, int *maxM, int *maxN) {
+                               const int32_t *begs, const int32_t *ends, int *maxM, int *maxN) {
   tm_status st;
   if ((st = reserve(g, nq)) != TM_OK) return st;
   Scratch &s = g->scratch;
-  long long acc = 0;
   *maxM = 2;
   *maxN = 2;
   for (int q = 0; q < nq; q++) {
@@ -577,28 +602,9 @@ static tm_status stage_queries(tm_graph *g, int nq, const int32_t *num_edges, co
     qd.root_end = end;
     qd.count = s.d_counts + q;
     qd.stop = s.d_stops + q;
-    s.h_prefix[q] = acc;
-    acc += ((long long)(end - beg) + chunk_roots - 1) / chunk_roots;
     *maxM = std::max(*maxM, qd.m.M);
     *maxN = std::max(*maxN, qd.m.n);
   }
-  s.h_prefix[nq] = acc;
-  *total_chunks = acc;
-  return TM_OK;
-}
-
-static int chunk_roots_of(const tm_tuning *t) {
-  int c = (t && t->chunk_roots > 0) ? t->chunk_roots : 256;
-  c = (c + 63) / 64 * 64;
-  return c;
-}
-
-static tm_status upload_staged(tm_graph *g, int nq) {
-  Scratch &s = g->scratch;
-  HIPTRY(hipMemcpyAsync(s.d_queries, s.h_queries.data(), sizeof(tm_query_dev) * nq, hipMemcpyHostToDevice,
-                        g->stream));
-  HIPTRY(hipMemcpyAsync(s.d_prefix, s.h_prefix.data(), sizeof(long long) * (nq + 1), hipMemcpyHostToDevice,
-                        g->stream));
   return TM_OK;
 }
 
@@ -621,14 +627,10 @@ tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const in
   if (!g || !msrc || !mdst || !count) return fail(TM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  const int cr = chunk_roots_of(tune);
-  long long chunks;
   int maxM, maxN;
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, cr, &chunks, &maxM,
-                               &maxN);
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, &maxM, &maxN);
   if (st != TM_OK) return st;
-  if ((st = upload_staged(g, 1)) != TM_OK) return st;
-  if ((st = launch(g, 1, maxM, maxN, false, tune, chunks, cr, false)) != TM_OK) return st;
+  if ((st = launch(g, 1, maxM, maxN, false, tune, false)) != TM_OK) return st;
   return finish(g, 1, count);
 }
 
@@ -637,14 +639,10 @@ tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, con
   if (!g || !msrc || !mdst || !count || !work) return fail(TM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  const int cr = chunk_roots_of(tune);
-  long long chunks;
   int maxM, maxN;
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, cr, &chunks, &maxM,
-                               &maxN);
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, &maxM, &maxN);
   if (st != TM_OK) return st;
-  if ((st = upload_staged(g, 1)) != TM_OK) return st;
-  if ((st = launch(g, 1, maxM, maxN, false, tune, chunks, cr, true)) != TM_OK) return st;
+  if ((st = launch(g, 1, maxM, maxN, false, tune, true)) != TM_OK) return st;
   if ((st = finish(g, 1, count)) != TM_OK) return st;
   const tm_query_dev &q = g->scratch.h_queries[0];
   work[0] = (uint64_t)(q.root_end - q.root_beg);
@@ -659,14 +657,11 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
   if (!g || nq < 1 || !num_edges || !msrc || !mdst || !deltas || !counts) return fail(TM_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  const int cr = chunk_roots_of(tune);
-  long long chunks;
   int maxM, maxN;
-  tm_status st = stage_queries(g, nq, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas, nullptr, nullptr, cr,
-                               &chunks, &maxM, &maxN);
+  tm_status st = stage_queries(g, nq, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas, nullptr, nullptr, &maxM,
+                               &maxN);
   if (st != TM_OK) return st;
-  if ((st = upload_staged(g, nq)) != TM_OK) return st;
-  if ((st = launch(g, nq, maxM, maxN, false, tune, chunks, cr, false)) != TM_OK) return st;
+  if ((st = launch(g, nq, maxM, maxN, false, tune, false)) != TM_OK) return st;
   return finish(g, nq, counts);
 }
 
@@ -676,11 +671,8 @@ tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int
   if (!g || !msrc || !mdst || !count || cap < 0 || (cap > 0 && !rows)) return fail(TM_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  const int cr = chunk_roots_of(tune);
-  long long chunks;
   int maxM, maxN;
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, cr, &chunks, &maxM,
-                               &maxN);
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, &maxM, &maxN);
   if (st != TM_OK) return st;
   Scratch &s = g->scratch;
   int32_t *d_rows = nullptr;
@@ -688,8 +680,7 @@ tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int
   HIPTRY(hipMalloc(&d_rows, row_bytes));
   s.h_queries[0].rows = d_rows;
   s.h_queries[0].cap = cap;
-  if ((st = upload_staged(g, 1)) != TM_OK) { (void)hipFree(d_rows); return st; }
-  if ((st = launch(g, 1, maxM, maxN, true, tune, chunks, cr, false)) != TM_OK) { (void)hipFree(d_rows); return st; }
+  if ((st = launch(g, 1, maxM, maxN, true, tune, false)) != TM_OK) { (void)hipFree(d_rows); return st; }
   st = finish(g, 1, count);
   if (st == TM_OK) {
     int64_t n = std::min<int64_t>((int64_t)*count, cap);

@@ -16,10 +16,9 @@ struct Scratch {
   unsigned long long *d_counts = nullptr;
   int *d_stops = nullptr;
   tm_sched *d_sched = nullptr;
-  tm_pool_frame *d_pool = nullptr;  // donated frames (work stealing between wavefronts)
-  int *d_pool_ready = nullptr;
+  unsigned long long *d_pool = nullptr;  // donated frames: TM_POOL_FIELDS tagged words each
   int pool_cap = 0;
-  int gen = 0;                      // launch generation for pool_ready tags
+  int gen = 0;                           // launch generation tag of pool words
   std::vector<tm_query_dev> h_queries;
   std::vector<long long> h_prefix;
 };

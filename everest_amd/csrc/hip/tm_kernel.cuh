@@ -99,18 +99,25 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// device-scope (all XCDs) atomics for the donation protocol
-__device__ __forceinline__ int ld_acq(const int *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Device-scope (all XCDs) relaxed atomics for the scheduling and donation
+// protocol.  Nothing here uses acquire/release orderings or fences: on a
+// multi-XCD part those write back / invalidate the XCD's L2, and thousands of
+// wavefronts doing that starve the busy ones.  Donated records carry a launch
+// generation tag in every word instead (see TM_POOL_FIELDS).
+__device__ __forceinline__ int ld_rlx(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ unsigned ld_acq(const unsigned *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned ld_rlx(const unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_rel(int *p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ int add_ar(int *p, int v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_rlx(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int add_rlx(int *p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // first position whose time is > tl (the window end as a position bound)
@@ -311,8 +318,10 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   int next_root = 0;       // first root of the next 64-root group (uniform)
   int root_end = 0;
   bool roots_done = false; // the root chunks of every query are handed out
+  const int home_stripe = blockIdx.x & (TM_STRIPES - 1);
+  int stripe_off = 0;
   bool is_idle = false;    // counted in sched->idle (else in sched->busy)
-  if (lane == 0) add_ar(&sched->busy, 1);
+  if (lane == 0) add_rlx(&sched->busy, 1);
 
   // copy one frame between the LDS stack (slot w) and a pool record, one field per lane
   auto frame_field = [&](int f, int w) -> int * {
@@ -331,7 +340,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     if ((++iter & 63u) == 0u) {  // deadline check (wall clock, uniform)
       if (args.limit_ticks > 0 && lane == 0 && (long long)wall_clock64() - sched->t0 > args.limit_ticks)
         atomicOr(&sched->abort, 1);
-      if (uni(*((volatile int *)&sched->abort))) {
+      if (uni(ld_rlx(&sched->abort))) {
         if (lane == 0) atomicOr(&sched->error, 4);
         break;
       }
@@ -339,52 +348,56 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     if (top == 0 && roots_done) {
       // ------------------------------------------ idle: take donated work
       if (!is_idle) {
-        if (lane == 0) { add_ar(&sched->busy, -1); add_ar(&sched->idle, 1); }
+        int nidle = 0;
+        if (lane == 0) { add_rlx(&sched->busy, -1); nidle = add_rlx(&sched->idle, 1) + 1; }
         is_idle = true;
+        // enough wavefronts are already waiting for donations: leave
+        if (uni(__shfl(nidle, 0)) > args.idle_cap) break;
       }
       int slot = -1;
-      for (int spin = 0; spin < (1 << 22); spin++) {
+      for (int spin = 0; spin < (1 << 20); spin++) {
         int h = 0, t = 0;
-        if (lane == 0) { h = (int)ld_acq(&sched->pool_head); t = (int)ld_acq(&sched->pool_tail); }
+        if (lane == 0) { h = (int)ld_rlx(&sched->pool_head); t = (int)ld_rlx(&sched->pool_tail); }
         h = uni(__shfl(h, 0));
         t = uni(__shfl(t, 0));
         if (h < t && h < args.pool_cap) {
           int won = 0;
           if (lane == 0) {
-            add_ar(&sched->busy, 1);
+            add_rlx(&sched->busy, 1);
             won = atomicCAS(&sched->pool_head, (unsigned)h, (unsigned)(h + 1)) == (unsigned)h;
-            if (!won) add_ar(&sched->busy, -1);
+            if (!won) add_rlx(&sched->busy, -1);
           }
           if (uni(__shfl(won, 0))) { slot = h; break; }
           continue;
         }
         int b = 0, ab = 0;
-        if (lane == 0) { b = ld_acq(&sched->busy); ab = ld_acq(&sched->abort); }
+        if (lane == 0) { b = ld_rlx(&sched->busy); ab = ld_rlx(&sched->abort); }
         if (uni(__shfl(ab, 0))) break;
         if (uni(__shfl(b, 0)) == 0) {
           // nobody holds work: anything reserved is already in the pool
-          if (lane == 0) { h = (int)ld_acq(&sched->pool_head); t = (int)ld_acq(&sched->pool_tail); }
+          if (lane == 0) { h = (int)ld_rlx(&sched->pool_head); t = (int)ld_rlx(&sched->pool_tail); }
           h = uni(__shfl(h, 0));
           t = uni(__shfl(t, 0));
           if (h < t && h < args.pool_cap) continue;
           break;
         }
-        __builtin_amdgcn_s_sleep(8);
+        __builtin_amdgcn_s_sleep(127);
       }
       if (slot < 0) break;
-      if (lane == 0) add_ar(&sched->idle, -1);
+      if (lane == 0) add_rlx(&sched->idle, -1);
       is_idle = false;
-      int rdy = 0;
-      if (lane == 0) {
-        for (int w = 0; w < (1 << 24); w++) {
-          if (ld_acq(&args.pool_ready[slot]) == args.gen) { rdy = 1; break; }
-          __builtin_amdgcn_s_sleep(1);
-        }
+      // read the record until every word carries this launch's generation
+      const unsigned long long *rec = args.pool + (size_t)slot * TM_POOL_FIELDS;
+      int fv = 0;
+      bool complete = false;
+      for (int w = 0; w < (1 << 22); w++) {
+        unsigned long long word = lane < TM_POOL_FIELDS ? ld_rlx(rec + lane) : 0ull;
+        const bool mine = lane >= TM_POOL_FIELDS || (int)(word >> 32) == args.gen;
+        fv = (int)(unsigned)(word & 0xffffffffull);
+        if (__ballot(!mine) == 0ull) { complete = true; break; }
+        __builtin_amdgcn_s_sleep(2);
       }
-      if (!uni(__shfl(rdy, 0))) { if (lane == 0) atomicOr(&sched->error, 8); break; }
-      __threadfence();
-      const int *rec = (const int *)&args.pool[slot];
-      const int fv = lane < TM_POOL_FIELDS ? rec[lane] : 0;
+      if (!complete) { if (lane == 0) atomicOr(&sched->error, 8); break; }
       const int fq = uni(__shfl(fv, 0));
       if (fq != qcur) {
         if (!ENUM && qcur >= 0) {
@@ -405,10 +418,20 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     if (top == 0) {
       // ---------------------------------------------------- root dispatch
       if (chunk_left == 0) {
-        long long c = 0;
-        if (lane == 0) c = (long long)atomicAdd(&sched->next_chunk, 1ull);
-        c = __shfl(c, 0);
-        if (c >= args.total_chunks) { roots_done = true; continue; }
+        // striped counters: stripe k hands out chunks [k C / S, (k+1) C / S);
+        // a wavefront starts on its own stripe and moves on when it runs dry
+        long long c = -1;
+        while (stripe_off < TM_STRIPES) {
+          const int k = (home_stripe + stripe_off) & (TM_STRIPES - 1);
+          const long long sb = args.total_chunks * k / TM_STRIPES;
+          const long long se = args.total_chunks * (k + 1) / TM_STRIPES;
+          long long o = 0;
+          if (lane == 0) o = (long long)atomicAdd(&sched->next_chunk[k][0], 1ull);
+          o = __shfl(o, 0);
+          if (sb + o < se) { c = sb + o; break; }
+          stripe_off++;
+        }
+        if (c < 0) { roots_done = true; continue; }
         // locate the query owning chunk c (uniform binary search)
         int lo = 0, hi = args.nq;
         while (hi - lo > 1) {
@@ -425,7 +448,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
           qcur = qi;
         }
         const tm_query_dev &q = args.queries[qcur];
-        if (ENUM && *((volatile int *)q.stop)) continue;
+        if (ENUM && ld_rlx(q.stop)) continue;
         long long first = (long long)q.root_beg + (c - args.chunk_prefix[qcur]) * (long long)args.chunk_roots;
         next_root = (int)first;
         root_end = q.root_end;
@@ -599,7 +622,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
               if (i < j) row[i] = g.Euid[e[i]];
             row[j] = g.Euid[pos];
           }
-          if ((long long)w + 1 >= q.cap) *((volatile int *)q.stop) = 1;
+          if ((long long)w + 1 >= q.cap) __hip_atomic_store(q.stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       } else {
         const int tm = md.tmode[j + 1];
@@ -646,7 +669,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     // ---------------------------------------------- donate to idle wavefronts
     if (top > 0 && (st_steps % (unsigned long long)args.steal_every) == 0) {
       int idl = 0;
-      if (lane == 0) idl = ld_acq(&sched->idle);
+      if (lane == 0) idl = ld_rlx(&sched->idle);
       idl = uni(__shfl(idl, 0));
       for (int d = 0; d < min(idl, args.steal_max); d++) {
         // split the shallowest frame with at least two candidates left
@@ -660,7 +683,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         if (t >= args.pool_cap) break;
         const int cur0 = f_cur[b], end0 = f_end[b];
         const int mid = cur0 + (end0 - cur0) / 2;
-        int *rec = (int *)&args.pool[t];
+        unsigned long long *rec = args.pool + (size_t)t * TM_POOL_FIELDS;
         if (lane < TM_POOL_FIELDS) {
           int v = 0;
           if (lane == 0) v = qcur;
@@ -669,12 +692,10 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
             int *src = frame_field(lane, b);
             v = src ? *src : 0;
           }
-          rec[lane] = v;
+          st_rlx(rec + lane, ((unsigned long long)(unsigned)args.gen << 32) | (unsigned)v);
         }
         __syncthreads();
         if (lane == 0) f_end[b] = mid;
-        __threadfence();
-        if (lane == 0) st_rel(&args.pool_ready[t], args.gen);
         __syncthreads();
       }
     }
@@ -685,8 +706,8 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     if (lane == 0 && s) atomicAdd(args.queries[qcur].count, s);
   }
   if (lane == 0) {
-    if (is_idle) add_ar(&sched->idle, -1);
-    else add_ar(&sched->busy, -1);
+    if (is_idle) add_rlx(&sched->idle, -1);
+    else add_rlx(&sched->busy, -1);
   }
   // work counters: one atomic per counter per wavefront
   unsigned long long w0 = wave_sum64(st_items), w1 = wave_sum64(st_roots), w2 = wave_sum64(st_leaves),

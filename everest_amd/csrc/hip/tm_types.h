@@ -84,8 +84,10 @@ typedef struct {
   int *stop;                  // set once cap rows are claimed: no more roots of this query
 } tm_query_dev;
 
+#define TM_STRIPES 8  // root-chunk counters, one 128-byte line each (one per XCD slot)
+
 typedef struct {
-  unsigned long long next_chunk;
+  unsigned long long next_chunk[TM_STRIPES][16];
   int error;        // bit 0: frame stack overflow, bit 1: empty frame, bit 2: deadline passed,
                     // bit 3: a donated frame never became ready
   int abort;        // set once the deadline passed: every wavefront stops
@@ -97,12 +99,11 @@ typedef struct {
   int idle;         // wavefronts waiting for donated work
 } tm_sched;
 
-// A frame donated by a busy wavefront to an idle one (global pool).
-typedef struct {
-  int q, lvl, cur, end, p, pend, tl;
-  int m[TM_MAXN];
-  int e[TM_MAXE];
-} tm_pool_frame;
+// A frame donated by a busy wavefront to an idle one (global pool): fields
+// q, lvl, cur, end, p, pend, tl, m[TM_MAXN], e[TM_MAXE], each stored as a
+// 64-bit word (launch generation << 32 | value) with device-scope atomic
+// stores, so a taker knows a record is complete when every word carries the
+// current generation: no fences, no cache invalidation.
 #define TM_POOL_FIELDS 23
 
 enum {
@@ -127,10 +128,10 @@ typedef struct {
   int scan_max;               // last-edge ranges up to this size are scanned
   long long limit_ticks;      // wall-clock budget of the launch (0 = none)
   tm_sched *sched;
-  tm_pool_frame *pool;        // donated frames
-  int *pool_ready;            // pool_ready[i] == gen once pool[i] is written
+  unsigned long long *pool;   // donated frames, TM_POOL_FIELDS tagged words each
   int pool_cap;
-  int gen;                    // launch generation tag for pool_ready
+  int gen;                    // launch generation tag of pool words
   int steal_every;            // busy wavefronts look for idle ones every this many steps
   int steal_max;              // frames donated per look
+  int idle_cap;               // most wavefronts kept waiting for donations
 } tm_kernel_args;

@@ -417,9 +417,12 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
 
 # ------------------------------------------------------------ many wavefronts
 
+STRIPES = 8
+
+
 class _Shared:
     def __init__(self, nchunks, pool_cap):
-        self.next_chunk = 0
+        self.next_chunk = [0] * STRIPES   # striped chunk counters
         self.nchunks = nchunks
         self.pool = []          # donated frames, in reservation order
         self.head = 0
@@ -430,7 +433,8 @@ class _Shared:
 
 class Wave:
     """One wavefront of mine_kernel as a step machine (one loop iteration per
-    step), sharing the chunk counter and the donation pool with the others."""
+    step), sharing the chunk counters and the donation pool with the others."""
+    count = 0
 
     def __init__(self, g, md, delta, beg, end, sh, scan_max, chunk_roots, steal_every, steal_max):
         self.g, self.md, self.delta, self.beg, self.end = g, md, delta, beg, end
@@ -447,6 +451,9 @@ class Wave:
         self.is_idle = False
         self.steps = 0
         self.done = False
+        self.home = Wave.count % STRIPES
+        Wave.count += 1
+        self.stripe_off = 0
         sh.busy += 1
 
     def step(self):
@@ -473,9 +480,17 @@ class Wave:
             return  # spin
         if self.top == 0:
             if self.chunk_left == 0:
-                c = sh.next_chunk
-                sh.next_chunk += 1
-                if c >= sh.nchunks:
+                c = -1
+                while self.stripe_off < STRIPES:
+                    k = (self.home + self.stripe_off) % STRIPES
+                    sb, se = sh.nchunks * k // STRIPES, sh.nchunks * (k + 1) // STRIPES
+                    o = sh.next_chunk[k]
+                    sh.next_chunk[k] += 1
+                    if sb + o < se:
+                        c = sb + o
+                        break
+                    self.stripe_off += 1
+                if c < 0:
                     self.roots_done = True
                     return
                 first = self.beg + c * self.chunk_roots
