@@ -35,7 +35,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_roots", ctypes.c_int32),
                 ("scan_max", ctypes.c_int32), ("time_limit_ms", ctypes.c_int32),
                 ("steal_every", ctypes.c_int32), ("steal_max", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("steal_min", ctypes.c_int32), ("steal_after_us", ctypes.c_int32)]
 
 
 _lib = None
@@ -78,10 +78,12 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def tuning(waves_per_cu=0, chunk_roots=0, scan_max=0, time_limit_ms=0, steal_every=0, steal_max=0):
+def tuning(waves_per_cu=0, chunk_roots=0, scan_max=0, time_limit_ms=0, steal_every=0, steal_max=0, steal_min=0,
+           steal_after_us=0):
     t = Tuning()
     t.waves_per_cu, t.chunk_roots, t.scan_max = waves_per_cu, chunk_roots, scan_max
-    t.time_limit_ms, t.steal_every, t.steal_max = time_limit_ms, steal_every, steal_max
+    t.time_limit_ms, t.steal_every, t.steal_max, t.steal_min = time_limit_ms, steal_every, steal_max, steal_min
+    t.steal_after_us = steal_after_us
     return t
 
 
@@ -118,9 +120,10 @@ class DeviceGraph:
         return dict(device_bytes=b.value, num_edges=e.value, num_vertices=v.value, device=d.value)
 
     def last_stats(self):
-        st = np.zeros(8, dtype=np.uint64)
-        check(lib().tm_last_stats(self.handle, _p(st), 8))
-        return dict(steps=int(st[0]), items=int(st[1]), roots=int(st[2]), leaves=int(st[3]), pushes=int(st[4]))
+        st = np.zeros(16, dtype=np.uint64)
+        check(lib().tm_last_stats(self.handle, _p(st), 16))
+        return dict(steps=int(st[0]), items=int(st[1]), roots=int(st[2]), leaves=int(st[3]), pushes=int(st[4]),
+                    donated=int(st[8]), taken=int(st[9]), capexit=int(st[10]), polls=int(st[11]))
 
     def last_kernel_ms(self):
         ms = ctypes.c_double()

@@ -144,7 +144,7 @@ __global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops
     s->idle = 0;
     s->t0 = (long long)wall_clock64();
   }
-  if (i < 8) s->stats[i] = 0;
+  if (i < 16) s->stats[i] = 0;
   if (i < n) { counts[i] = 0; stops[i] = 0; }
 }
 
@@ -171,7 +171,7 @@ struct tm_graph {
   float last_kernel_ms = 0.f;
   int last_chunk_roots = 0, last_grid = 0;
   tmh::Scratch scratch;
-  unsigned long long last_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long last_stats[16] = {};
   std::mutex mtx;  // one launch at a time per graph handle
 };
 
@@ -323,7 +323,12 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   a.gen = ++s.gen;
   a.steal_every = (tune && tune->steal_every > 0) ? tune->steal_every : 16;
   a.steal_max = (tune && tune->steal_max > 0) ? tune->steal_max : 4;
-  a.idle_cap = std::max(64, g->num_cus * 2);
+  a.idle_cap = std::max(64, g->num_cus / 2);
+  a.steal_min = (tune && tune->steal_min > 0) ? tune->steal_min : 64;
+  {
+    const long long us = (tune && tune->steal_after_us > 0) ? tune->steal_after_us : 200;
+    a.steal_after_ticks = us * g->wall_rate_khz / 1000;
+  }
   g->last_chunk_roots = chunk_roots;
   g->last_grid = grid;
 
@@ -557,7 +562,7 @@ tm_status tm_graph_free(tm_graph *g) {
 
 tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n) {
   if (!g || !stats || n < 0) return fail(TM_ERR_ARG, "bad argument");
-  for (int i = 0; i < n && i < 8; i++) stats[i] = g->last_stats[i];
+  for (int i = 0; i < n && i < 16; i++) stats[i] = g->last_stats[i];
   return TM_OK;
 }
 

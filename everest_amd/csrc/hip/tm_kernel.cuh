@@ -311,6 +311,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   unsigned long long cnt = 0;
   unsigned long long st_steps = 0, st_items = 0, st_roots = 0, st_leaves = 0, st_push = 0;
   unsigned long long rx1 = 0, rx2 = 0, rp = 0;  // reference work (INSTR only)
+  unsigned long long st_don = 0, st_take = 0, st_capexit = 0, st_polls = 0;
   unsigned iter = 0;
   int qcur = -1;
   int top = 0;
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   bool roots_done = false; // the root chunks of every query are handed out
   const int home_stripe = blockIdx.x & (TM_STRIPES - 1);
   int stripe_off = 0;
+  long long task_t0 = 0;   // wall clock when the current task (root group or donated frame) began
   bool is_idle = false;    // counted in sched->idle (else in sched->busy)
   if (lane == 0) add_rlx(&sched->busy, 1);
 
@@ -352,10 +354,11 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         if (lane == 0) { add_rlx(&sched->busy, -1); nidle = add_rlx(&sched->idle, 1) + 1; }
         is_idle = true;
         // enough wavefronts are already waiting for donations: leave
-        if (uni(__shfl(nidle, 0)) > args.idle_cap) break;
+        if (uni(__shfl(nidle, 0)) > args.idle_cap) { st_capexit++; break; }
       }
       int slot = -1;
       for (int spin = 0; spin < (1 << 20); spin++) {
+        st_polls++;
         int h = 0, t = 0;
         if (lane == 0) { h = (int)ld_rlx(&sched->pool_head); t = (int)ld_rlx(&sched->pool_tail); }
         h = uni(__shfl(h, 0));
@@ -368,6 +371,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
             if (!won) add_rlx(&sched->busy, -1);
           }
           if (uni(__shfl(won, 0))) { slot = h; break; }
+          __builtin_amdgcn_s_sleep(16);  // lost the race: back off before polling again
           continue;
         }
         int b = 0, ab = 0;
@@ -384,6 +388,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         __builtin_amdgcn_s_sleep(127);
       }
       if (slot < 0) break;
+      st_take++;
       if (lane == 0) add_rlx(&sched->idle, -1);
       is_idle = false;
       // read the record until every word carries this launch's generation
@@ -412,6 +417,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         if (dst) *dst = fv;
       }
       top = 1;
+      task_t0 = (long long)wall_clock64();
       __syncthreads();
       continue;
     }
@@ -519,6 +525,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         if constexpr (ENUM) f_e[w] = r;
       }
       top += n;
+      task_t0 = (long long)wall_clock64();
       __syncthreads();
       continue;
     }
@@ -667,13 +674,23 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     top = ntop + n;
     __syncthreads();
     // ---------------------------------------------- donate to idle wavefronts
-    if (top > 0 && (st_steps % (unsigned long long)args.steal_every) == 0) {
-      int idl = 0;
-      if (lane == 0) idl = ld_rlx(&sched->idle);
-      idl = uni(__shfl(idl, 0));
-      for (int d = 0; d < min(idl, args.steal_max); d++) {
-        // split the shallowest frame with at least two candidates left
-        const bool can = lane < top && (f_end[lane] - f_cur[lane] >= 2);
+    // only a wavefront that has held its current task for steal_after_ticks
+    // donates: short tasks never pay for the exchange
+    if (top > 0 && (st_steps % (unsigned long long)args.steal_every) == 0 &&
+        (long long)wall_clock64() - task_t0 > args.steal_after_ticks) {
+      // donate only against unmet demand: waiting wavefronts minus frames
+      // already waiting in the pool
+      int want = 0;
+      if (lane == 0) {
+        const int idl = ld_rlx(&sched->idle);
+        const int hd = (int)ld_rlx(&sched->pool_head);
+        const int tl = min((int)ld_rlx(&sched->pool_tail), args.pool_cap);
+        want = min(idl - (tl - hd), args.steal_max);
+      }
+      want = uni(__shfl(want, 0));
+      for (int d = 0; d < want; d++) {
+        // split the shallowest frame with at least 2 * steal_min candidates left
+        const bool can = lane < top && (f_end[lane] - f_cur[lane] >= 2 * args.steal_min);
         const unsigned long long cm = __ballot(can);
         if (cm == 0ull) break;
         const int b = __ffsll((long long)cm) - 1;
@@ -696,6 +713,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         }
         __syncthreads();
         if (lane == 0) f_end[b] = mid;
+        st_don++;
         __syncthreads();
       }
     }
@@ -718,6 +736,10 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     atomicAdd(&sched->stats[TM_STAT_ROOTS], w1);
     atomicAdd(&sched->stats[TM_STAT_LEAVES], w2);
     atomicAdd(&sched->stats[TM_STAT_PUSHES], w3);
+    if (st_don) atomicAdd(&sched->stats[TM_STAT_DONATED], st_don);
+    if (st_take) atomicAdd(&sched->stats[TM_STAT_TAKEN], st_take);
+    if (st_capexit) atomicAdd(&sched->stats[TM_STAT_CAPEXIT], st_capexit);
+    atomicAdd(&sched->stats[TM_STAT_POLLS], st_polls);
   }
   if constexpr (INSTR) {
     unsigned long long a = wave_sum64(rx1), b = wave_sum64(rx2), c = wave_sum64(rp);

@@ -92,7 +92,7 @@ typedef struct {
                     // bit 3: a donated frame never became ready
   int abort;        // set once the deadline passed: every wavefront stops
   long long t0;     // wall clock at launch (set by the reset kernel)
-  unsigned long long stats[8];  // work counters (see TM_STAT_*)
+  unsigned long long stats[16];  // work counters (see TM_STAT_*)
   unsigned int pool_head;  // next donated frame to take
   unsigned int pool_tail;  // donated frames reserved so far
   int busy;         // wavefronts holding work (may still donate)
@@ -114,7 +114,11 @@ enum {
   TM_STAT_PUSHES = 4,  // frames pushed
   TM_STAT_REF_X1 = 5,  // reference list resolutions, new-node edges (instrumented run)
   TM_STAT_REF_X2 = 6,  // reference list resolutions, closing edges (instrumented run)
-  TM_STAT_REF_P = 7    // reference candidate probes (instrumented run)
+  TM_STAT_REF_P = 7,   // reference candidate probes (instrumented run)
+  TM_STAT_DONATED = 8, // frames donated to the pool
+  TM_STAT_TAKEN = 9,   // donated frames taken
+  TM_STAT_CAPEXIT = 10,// wavefronts that left because enough were waiting
+  TM_STAT_POLLS = 11   // pool polls by waiting wavefronts
 };
 
 typedef struct {
@@ -134,4 +138,6 @@ typedef struct {
   int steal_every;            // busy wavefronts look for idle ones every this many steps
   int steal_max;              // frames donated per look
   int idle_cap;               // most wavefronts kept waiting for donations
+  int steal_min;              // a donated half holds at least this many candidates
+  long long steal_after_ticks; // a task donates only after running this long (wall clock)
 } tm_kernel_args;

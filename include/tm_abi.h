@@ -66,7 +66,8 @@ typedef struct tm_tuning {
   int32_t time_limit_ms; /* abort the launch after this wall time on the device (0 = no limit) */
   int32_t steal_every;   /* busy wavefronts look for idle ones every this many steps */
   int32_t steal_max;     /* frames a busy wavefront donates per look */
-  int32_t reserved[2];
+  int32_t steal_min;     /* a donated half-frame holds at least this many candidates */
+  int32_t steal_after_us; /* a wavefront donates only after holding one task this long */
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */
@@ -91,7 +92,10 @@ tm_status tm_graph_free(tm_graph *g);
 /*
  * Work counters of the graph's last launch: [0] wavefront steps, [1] candidate
  * items examined, [2] roots dispatched, [3] last-edge range computations,
- * [4] frames pushed.  No reference counterpart (instrumentation).
+ * [4] frames pushed, [5..7] reference work of an instrumented run (X1, X2, P),
+ * [8] frames donated between wavefronts, [9] donated frames taken, [10]
+ * wavefronts that left early, [11] pool polls.  Up to 16 entries.
+ * No reference counterpart (instrumentation).
  */
 tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n);
 

@@ -436,10 +436,10 @@ class Wave:
     step), sharing the chunk counters and the donation pool with the others."""
     count = 0
 
-    def __init__(self, g, md, delta, beg, end, sh, scan_max, chunk_roots, steal_every, steal_max):
+    def __init__(self, g, md, delta, beg, end, sh, scan_max, chunk_roots, steal_every, steal_max, steal_min=1):
         self.g, self.md, self.delta, self.beg, self.end = g, md, delta, beg, end
         self.sh, self.scan_max, self.chunk_roots = sh, scan_max, chunk_roots
-        self.steal_every, self.steal_max = steal_every, steal_max
+        self.steal_every, self.steal_max, self.steal_min = steal_every, steal_max, steal_min
         M = md["M"]
         self.FR = WAVE * max(1, M - 2)
         self.frames = [None] * self.FR
@@ -595,8 +595,9 @@ class Wave:
         self.steps += 1
         # donation
         if self.top > 0 and self.steps % self.steal_every == 0:
-            for _ in range(min(sh.idle, self.steal_max)):
-                cand = [i for i in range(min(self.top, WAVE)) if frames[i]["end"] - frames[i]["cur"] >= 2]
+            want = min(sh.idle - (min(len(sh.pool), sh.cap) - sh.head), self.steal_max)
+            for _ in range(max(0, want)):
+                cand = [i for i in range(min(self.top, WAVE)) if frames[i]["end"] - frames[i]["cur"] >= 2 * self.steal_min]
                 if not cand:
                     break
                 b = cand[0]
