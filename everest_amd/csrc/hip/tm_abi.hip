@@ -265,7 +265,8 @@ static tm_status dispatch(tm_graph *g, int nmap, bool enumerate, bool instr, int
 // Plan and launch the mining kernel for nq queries staged in scratch.h_queries:
 // size the persistent grid from occupancy, pick the chunk size, lay the
 // queries' root chunks out in one index space, upload, reset, launch.
-tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const tm_tuning *tune, bool instr) {
+tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const tm_tuning *tune, bool instr,
+                 tm_launch_fn fn = nullptr, void *user = nullptr) {
   Scratch &s = g->scratch;
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
@@ -273,8 +274,14 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   const int per_frame = 6 + nmap + (enumerate ? TM_MAXE : 0);
   const int lds = frames * per_frame * (int)sizeof(int);
   int blocks = 0;
-  tm_status st = dispatch<true>(g, nmap, enumerate, instr, lds, &blocks, nullptr, 0);
-  if (st != TM_OK) return st;
+  tm_status st;
+  if (fn) {
+    int32_t b = 0;
+    if (fn(0, nullptr, 0, lds, g->stream, &b, user) != 0) return fail(TM_ERR_HIP, "custom kernel occupancy query failed");
+    blocks = b;
+  } else if ((st = dispatch<true>(g, nmap, enumerate, instr, lds, &blocks, nullptr, 0)) != TM_OK) {
+    return st;
+  }
   int wpc = (tune && tune->waves_per_cu > 0) ? tune->waves_per_cu : 32;
   if (blocks > 0) wpc = std::min(wpc, blocks);
   if (wpc < 1) wpc = 1;
@@ -335,6 +342,12 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, g->stream, s.d_sched,
                      s.d_counts, s.d_stops, nq);
   HIPTRY(hipGetLastError());
+  if (fn) {
+    HIPTRY(hipEventRecord(g->ev0, g->stream));
+    if (fn(1, &a, grid, lds, g->stream, nullptr, user) != 0) return fail(TM_ERR_HIP, "custom kernel launch failed");
+    HIPTRY(hipEventRecord(g->ev1, g->stream));
+    return TM_OK;
+  }
   return dispatch<false>(g, nmap, enumerate, instr, lds, &blocks, &a, grid);
 }
 
@@ -627,16 +640,29 @@ static tm_status finish(tm_graph *g, int nq, uint64_t *counts) {
   return TM_OK;
 }
 
-tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
-                   int32_t beg, int32_t end, const tm_tuning *tune, uint64_t *count) {
+static tm_status count_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                            int32_t beg, int32_t end, const tm_tuning *tune, uint64_t *count, tm_launch_fn fn,
+                            void *user) {
   if (!g || !msrc || !mdst || !count) return fail(TM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
   int maxM, maxN;
   tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, &maxM, &maxN);
   if (st != TM_OK) return st;
-  if ((st = launch(g, 1, maxM, maxN, false, tune, false)) != TM_OK) return st;
+  if ((st = launch(g, 1, maxM, maxN, false, tune, false, fn, user)) != TM_OK) return st;
   return finish(g, 1, count);
+}
+
+tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                   int32_t beg, int32_t end, const tm_tuning *tune, uint64_t *count) {
+  return count_impl(g, num_edges, msrc, mdst, delta, beg, end, tune, count, nullptr, nullptr);
+}
+
+tm_status tm_count_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                          int32_t beg, int32_t end, const tm_tuning *tune, tm_launch_fn launcher, void *user,
+                          uint64_t *count) {
+  if (!launcher) return fail(TM_ERR_ARG, "null launcher");
+  return count_impl(g, num_edges, msrc, mdst, delta, beg, end, tune, count, launcher, user);
 }
 
 tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
@@ -670,9 +696,26 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
   return finish(g, nq, counts);
 }
 
+static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                           int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows,
+                           uint64_t *count, int64_t *rows_written, tm_launch_fn fn, void *user);
+
 tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
                   int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows, uint64_t *count,
                   int64_t *rows_written) {
+  return enum_impl(g, num_edges, msrc, mdst, delta, beg, end, cap, tune, rows, count, rows_written, nullptr, nullptr);
+}
+
+tm_status tm_enum_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                         int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, tm_launch_fn launcher,
+                         void *user, int32_t *rows, uint64_t *count, int64_t *rows_written) {
+  if (!launcher) return fail(TM_ERR_ARG, "null launcher");
+  return enum_impl(g, num_edges, msrc, mdst, delta, beg, end, cap, tune, rows, count, rows_written, launcher, user);
+}
+
+static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                           int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows,
+                           uint64_t *count, int64_t *rows_written, tm_launch_fn fn, void *user) {
   if (!g || !msrc || !mdst || !count || cap < 0 || (cap > 0 && !rows)) return fail(TM_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
@@ -685,7 +728,7 @@ tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int
   HIPTRY(hipMalloc(&d_rows, row_bytes));
   s.h_queries[0].rows = d_rows;
   s.h_queries[0].cap = cap;
-  if ((st = launch(g, 1, maxM, maxN, true, tune, false)) != TM_OK) { (void)hipFree(d_rows); return st; }
+  if ((st = launch(g, 1, maxM, maxN, true, tune, false, fn, user)) != TM_OK) { (void)hipFree(d_rows); return st; }
   st = finish(g, 1, count);
   if (st == TM_OK) {
     int64_t n = std::min<int64_t>((int64_t)*count, cap);

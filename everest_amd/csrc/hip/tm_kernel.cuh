@@ -120,6 +120,29 @@ __device__ __forceinline__ int add_rlx(int *p, int v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Motif-shape policy of the kernel.  DynSpec reads the per-query descriptor
+// (one kernel serves any mix of queries: batches, serving); a generated spec
+// returns compile-time constants for one motif, so the compiler folds the
+// per-edge branches away.
+struct DynSpec {
+  static __device__ __forceinline__ int M(const tm_motif_dev &md) { return md.M; }
+  static __device__ __forceinline__ int kind(const tm_motif_dev &md, int j) { return md.kind[j]; }
+  static __device__ __forceinline__ int a(const tm_motif_dev &md, int j) { return md.a[j]; }
+  static __device__ __forceinline__ int b(const tm_motif_dev &md, int j) { return md.b[j]; }
+  static __device__ __forceinline__ int mapped(const tm_motif_dev &md, int j) { return md.mapped[j]; }
+  static __device__ __forceinline__ int smode(const tm_motif_dev &md, int j) { return md.smode[j]; }
+  static __device__ __forceinline__ int tmode(const tm_motif_dev &md, int j) { return md.tmode[j]; }
+};
+
+// element j of a compile-time table (j uniform: folds to scalar selects)
+template <int N>
+__device__ __forceinline__ int pick(const int (&t)[N], int j) {
+  int v = t[0];
+#pragma unroll
+  for (int i = 1; i < N; i++) v = (i == j) ? t[i] : v;
+  return v;
+}
+
 // first position whose time is > tl (the window end as a position bound)
 __device__ __forceinline__ int pend_of(const tm_graph_view &g, long long tl) {
   long long k = (tl - (long long)g.tmin) >> g.bshift;
@@ -186,15 +209,15 @@ struct CandList {
   int lo, hi, dir;
 };
 
-template <int MAXN>
+template <class SP, int MAXN>
 __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_motif_dev &md, int j,
                                                const int (&m)[MAXN], int p) {
   CandList L;
   L.dir = 0;
-  const int kind = md.kind[j];
-  const int mode = md.smode[j];
+  const int kind = SP::kind(md, j);
+  const int mode = SP::smode(md, j);
   if (kind == TM_K_NEW_DST) {
-    const int x = sel(m, md.a[j]);
+    const int x = sel(m, SP::a(md, j));
     L.pos = g.outPos;
     L.nbr = g.outNbr;
     L.hi = g.outR[x + 1];
@@ -202,7 +225,7 @@ __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_
     else if (mode == TM_S_OUT_NEXT) L.lo = g.outNext[p];
     else L.lo = upper_bound_i(g.outPos, g.outR[x], L.hi, p);
   } else if (kind == TM_K_NEW_SRC) {
-    const int y = sel(m, md.b[j]);
+    const int y = sel(m, SP::b(md, j));
     L.pos = g.inPos;
     L.nbr = g.inNbr;
     L.hi = g.inR[y + 1];
@@ -210,7 +233,7 @@ __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_
     else if (mode == TM_S_IN_NEXT) L.lo = g.inNext[p];
     else L.lo = upper_bound_i(g.inPos, g.inR[y], L.hi, p);
   } else {
-    const int x = sel(m, md.a[j]), y = sel(m, md.b[j]);
+    const int x = sel(m, SP::a(md, j)), y = sel(m, SP::b(md, j));
     L.nbr = nullptr;
     if (mode == TM_S_PAIR_SELF) {
       // the same pair as edge j-1: continue right after p inside its segment
@@ -240,22 +263,22 @@ __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_
 
 // Matches of the final motif edge j given the mapping through edge j-1 and the
 // list from cand_start, counted without enumerating them.
-template <int MAXN>
+template <class SP, int MAXN>
 __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g, const tm_motif_dev &md,
                                                          int j, const int (&m)[MAXN], const CandList &L,
                                                          int p, int pend, int scan_max) {
   const int b1 = gallop_lb(L.pos, L.lo, L.hi, pend);
   int c = b1 - L.lo;
-  if (c <= 0 || md.kind[j] == TM_K_CLOSE) return c > 0 ? (unsigned long long)c : 0ull;
-  const int nm = md.mapped[j];
+  if (c <= 0 || SP::kind(md, j) == TM_K_CLOSE) return c > 0 ? (unsigned long long)c : 0ull;
+  const int nm = SP::mapped(md, j);
   if (c <= scan_max) {
     int bad = 0;
     for (int s = L.lo; s < b1; s++) bad += in_first(m, nm, L.nbr[s]) ? 1 : 0;
     return (unsigned long long)(c - bad);
   }
   // injectivity: remove candidates whose new endpoint is already mapped
-  const bool out = md.kind[j] == TM_K_NEW_DST;
-  const int base = out ? sel(m, md.a[j]) : sel(m, md.b[j]);
+  const bool out = SP::kind(md, j) == TM_K_NEW_DST;
+  const int base = out ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j));
 #pragma unroll
   for (int i = 0; i < MAXN; i++)
     if (i < nm) c -= out ? edge_count(g, base, m[i], p, pend) : edge_count(g, m[i], base, p, pend);
@@ -271,18 +294,18 @@ __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g,
 // the reference scans list entries after p until the first one outside the
 // window, and that terminating probe counts too.  For a closing edge it scans
 // the shorter of in(m[b]) and out(m[a]) (ties to out).
-template <int MAXN>
+template <class SP, int MAXN>
 __device__ __forceinline__ unsigned long long ref_probes(const tm_graph_view &g, const tm_motif_dev &md, int k,
                                                          const int (&m)[MAXN], int p, int pend) {
-  const int kind = md.kind[k];
+  const int kind = SP::kind(md, k);
   const int *pos;
   int lo, hi;
-  if (kind == TM_K_NEW_DST || (kind == TM_K_CLOSE && !(g.inR[sel(m, md.b[k]) + 1] - g.inR[sel(m, md.b[k])] <
-                                                        g.outR[sel(m, md.a[k]) + 1] - g.outR[sel(m, md.a[k])]))) {
-    const int x = sel(m, md.a[k]);
+  if (kind == TM_K_NEW_DST || (kind == TM_K_CLOSE && !(g.inR[sel(m, SP::b(md, k)) + 1] - g.inR[sel(m, SP::b(md, k))] <
+                                                        g.outR[sel(m, SP::a(md, k)) + 1] - g.outR[sel(m, SP::a(md, k))]))) {
+    const int x = sel(m, SP::a(md, k));
     pos = g.outPos; lo = g.outR[x]; hi = g.outR[x + 1];
   } else {
-    const int y = sel(m, md.b[k]);
+    const int y = sel(m, SP::b(md, k));
     pos = g.inPos; lo = g.inR[y]; hi = g.inR[y + 1];
   }
   lo = upper_bound_i(pos, lo, hi, p);
@@ -290,7 +313,7 @@ __device__ __forceinline__ unsigned long long ref_probes(const tm_graph_view &g,
   return (unsigned long long)(e - lo) + (e < hi ? 1ull : 0ull);
 }
 
-template <int MAXE, int MAXN, bool ENUM, bool INSTR>
+template <int MAXE, int MAXN, bool ENUM, bool INSTR, class SP = DynSpec>
 __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   extern __shared__ int lds[];
   const int lane = threadIdx.x;
@@ -465,7 +488,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       }
       const tm_query_dev &q = args.queries[qcur];
       const tm_motif_dev &md = q.m;
-      const int M = md.M;
+      const int M = SP::M(md);
       const int r = next_root + lane;
       next_root += WAVE;
       chunk_left--;
@@ -485,17 +508,17 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       CandList L{g.outPos, g.outNbr, 0, 0, 0};
       const long long tl = (long long)t + q.delta;
       const int tli = tl > 0x7fffffffll ? 0x7fffffff : (int)tl;
-      if (ok && md.tmode[1] >= 0) ok = g.tfirst[md.tmode[1]][r] <= tli;  // streamed, coalesced
+      if (ok && SP::tmode(md, 1) >= 0) ok = g.tfirst[SP::tmode(md, 1)][r] <= tli;  // streamed, coalesced
       if (ok) {
-        L = cand_start(g, md, 1, m, r);
+        L = cand_start<SP>(g, md, 1, m, r);
         ok = L.lo < L.hi && (long long)g.T[L.pos[L.lo]] <= tl;
       }
       int pend = 0;
       if constexpr (INSTR) {
         if (r < root_end && u != v) {
           const int pe = pend_of(g, tl);
-          (md.kind[1] == TM_K_CLOSE ? rx2 : rx1) += 1;
-          rp += ref_probes(g, md, 1, m, r, pe);
+          (SP::kind(md, 1) == TM_K_CLOSE ? rx2 : rx1) += 1;
+          rp += ref_probes<SP>(g, md, 1, m, r, pe);
         }
       }
       if (ok) {
@@ -505,7 +528,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       const int last = ENUM ? M - 1 : M - 2;  // last motif edge handled per candidate
       if (last == 0) {
         // count mode, M == 2: edge 1 is the final edge
-        if (ok) { cnt += leaf_count(g, md, 1, m, L, r, pend, args.scan_max); st_leaves++; }
+        if (ok) { cnt += leaf_count<SP>(g, md, 1, m, L, r, pend, args.scan_max); st_leaves++; }
         continue;
       }
       int cb = 0, ce = 0, dir = L.dir;
@@ -533,7 +556,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     // ------------------------------------------------------ expansion step
     const tm_query_dev &q = args.queries[qcur];
     const tm_motif_dev &md = q.m;
-    const int M = md.M;
+    const int M = SP::M(md);
     const int L = uni(f_lvl[top - 1] & 0xff);
     const int fi = top - 1 - lane;
     int fcnt = 0;
@@ -593,19 +616,19 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     int ntop = top - nparts + (last_partial ? 1 : 0);
 
     const int j = L + 1;  // motif edge of the candidates
-    const int kind = md.kind[j];
+    const int kind = SP::kind(md, j);
     bool valid = false;
     int pos = 0;
     if (has) {
       int node = 0;
       if (kind == TM_K_NEW_DST) {
         pos = g.outPos[slot]; node = g.outNbr[slot];
-        valid = !in_first(m, md.mapped[j], node);
-        put(m, md.b[j], node);
+        valid = !in_first(m, SP::mapped(md, j), node);
+        put(m, SP::b(md, j), node);
       } else if (kind == TM_K_NEW_SRC) {
         pos = g.inPos[slot]; node = g.inNbr[slot];
-        valid = !in_first(m, md.mapped[j], node);
-        put(m, md.a[j], node);
+        valid = !in_first(m, SP::mapped(md, j), node);
+        put(m, SP::a(md, j), node);
       } else {
         pos = dir ? g.inPairPos[slot] : g.outPairPos[slot];
         valid = true;
@@ -614,8 +637,8 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     const int last = ENUM ? M - 1 : M - 2;
     if constexpr (INSTR) {
       if (valid && j + 1 < M) {
-        (md.kind[j + 1] == TM_K_CLOSE ? rx2 : rx1) += 1;
-        rp += ref_probes(g, md, j + 1, m, pos, pend);
+        (SP::kind(md, j + 1) == TM_K_CLOSE ? rx2 : rx1) += 1;
+        rp += ref_probes<SP>(g, md, j + 1, m, pos, pend);
       }
     }
     if (j == last) {
@@ -632,10 +655,10 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
           if ((long long)w + 1 >= q.cap) __hip_atomic_store(q.stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       } else {
-        const int tm = md.tmode[j + 1];
+        const int tm = SP::tmode(md, j + 1);
         if (valid && (tm < 0 || g.tfirst[tm][pos] <= tl)) {
-          const CandList LL = cand_start(g, md, j + 1, m, pos);
-          cnt += leaf_count(g, md, j + 1, m, LL, pos, pend, args.scan_max);
+          const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
+          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max);
           st_leaves++;
         }
       }
@@ -645,11 +668,11 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     }
     int cb = 0, ce = 0, cdir = 0;
     {
-      const int tm = md.tmode[j + 1];
+      const int tm = SP::tmode(md, j + 1);
       if (valid && tm >= 0) valid = g.tfirst[tm][pos] <= tl;
     }
     if (valid) {
-      const CandList LL = cand_start(g, md, j + 1, m, pos);
+      const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
       cb = LL.lo;
       cdir = LL.dir;
       ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend);

@@ -1,0 +1,121 @@
+"""Per-query worker generator (reference: system/tmine/KernelGen.py:1-538).
+
+The reference expands CUDA snippets into a motif-specialised sm_86 worker.
+Here the generator fills templates/worker.hip.tmpl: a gfx950 HIP plugin that
+instantiates the library's mining kernel template with the motif as
+compile-time tables (kinds, node slots, list start rules, first-candidate
+time arrays per edge), the per-query options baked into its launch
+parameters, and the reference worker entry point getWorker.
+"""
+import os
+from string import Template
+
+from . import locations
+
+K_NEW_DST, K_NEW_SRC, K_CLOSE = 0, 1, 2
+S_SEARCH, S_OUT_SELF, S_OUT_NEXT, S_IN_SELF, S_IN_NEXT, S_PAIR_SELF = range(6)
+MAXE = 8
+MAXN = 8
+
+
+def InspectMotif(pathToMotif):
+    """(mapped node count before each edge >= 1, max stack size, edge count),
+    from the motif file in line order (KernelGen.py:13-29)."""
+    mnode, mnode_p = [], []
+    allE = 0
+    with open(pathToMotif, "r") as f:
+        for line in f:
+            if not line.split():
+                continue
+            allE += 1
+            u, v, _ = line.split()[:3]
+            mnode_p.append(len(mnode))
+            if u not in mnode:
+                mnode.append(u)
+            if v not in mnode:
+                mnode.append(v)
+    return mnode_p[1:], allE - 2, allE
+
+
+def motif_edges(pathToMotif):
+    """Renamed, time-ordered (src, dst) pairs, read by the core library's motif
+    reader (the same reader HostMotifData uses)."""
+    from everest_amd import core
+    m = core.data.HostMotifData(pathToMotif)
+    return [(u, v) for (u, v, _) in m.edges()]
+
+
+def motif_tables(edges):
+    """Per-edge tables of the kernel spec, as the library builds them at run time."""
+    M = len(edges)
+    if M < 2 or M > MAXE:
+        raise ValueError("motifs need 2..%d edges" % MAXE)
+    if edges[0] != (0, 1):
+        raise ValueError("motif nodes must be renamed by first appearance")
+    kind, A, B, mapped, smode, tmode = ([0] * MAXE for _ in range(6))
+    tmode = [-1] * MAXE
+    n = 2
+    for j in range(1, M):
+        a, b = edges[j]
+        pa, pb = edges[j - 1]
+        if a == b:
+            raise ValueError("motif self-loop")
+        A[j], B[j], mapped[j] = a, b, n
+        if a < n and b < n:
+            kind[j] = K_CLOSE
+            smode[j] = S_PAIR_SELF if (a, b) == (pa, pb) else S_SEARCH
+            tmode[j] = 4 if (a, b) == (pa, pb) else (5 if (a, b) == (pb, pa) else -1)
+        elif a < n and b == n:
+            kind[j] = K_NEW_DST
+            smode[j] = S_OUT_SELF if a == pa else (S_OUT_NEXT if a == pb else S_SEARCH)
+            tmode[j] = 0 if a == pa else (1 if a == pb else -1)
+            n += 1
+        elif b < n and a == n:
+            kind[j] = K_NEW_SRC
+            smode[j] = S_IN_SELF if b == pb else (S_IN_NEXT if b == pa else S_SEARCH)
+            tmode[j] = 2 if b == pb else (3 if b == pa else -1)
+            n += 1
+        else:
+            raise ValueError("motif is not prefix-connected")
+        if n > MAXN:
+            raise ValueError("motif has more than %d nodes" % MAXN)
+    return dict(M=M, N=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode)
+
+
+class KernelGen:
+    def __init__(self, pathToMotif, query, tuning=None):
+        self.query = query
+        self.options = query.options()
+        self.cons = query.constraints()
+        feats = self.cons.features()
+        if feats:
+            raise NotImplementedError("constraint features %s are not supported by this build "
+                                      "(the delta-window query path is)" % feats)
+        if self.options.multigpu():
+            raise NotImplementedError("this build drives one GPU (options.gpu = %d)" % self.options.gpu())
+        self.pathToMotif = pathToMotif
+        self.edges = motif_edges(pathToMotif)
+        self.tables = motif_tables(self.edges)
+        self.enumerate = self.options.enumerate()
+        self.enum_NUM = self.options.enumNum() if self.enumerate else -1
+        t = dict(waves_per_cu=0, chunk_roots=0, scan_max=0, steal_every=0, steal_max=0, steal_min=0,
+                 steal_after_us=0)
+        if tuning:
+            t.update({k: int(v) for k, v in tuning.items() if k in t})
+        if not self.options.opts.get("warpBalancing", True):
+            t["steal_every"] = 1 << 30  # no work donation between wavefronts
+        self.tuning = t
+
+    def generate(self):
+        tb = self.tables
+        with open(os.path.join(locations.templateRoot(), "worker.hip.tmpl")) as f:
+            tmpl = Template(f.read())
+        join = lambda xs: ", ".join(str(int(x)) for x in xs)
+        subs = dict(motif_name=os.path.basename(self.pathToMotif), M=tb["M"], N=tb["N"],
+                    options=", ".join("%s=%s" % kv for kv in sorted(self.options.opts.items())),
+                    src=join(a for a, _ in self.edges), dst=join(b for _, b in self.edges),
+                    enum_cap=self.enum_NUM, kind=join(tb["kind"]), a=join(tb["a"]), b=join(tb["b"]),
+                    mapped=join(tb["mapped"]), smode=join(tb["smode"]), tmode=join(tb["tmode"]),
+                    nmap=4 if tb["N"] <= 4 else 8)
+        subs.update(self.tuning)
+        return tmpl.substitute(subs)

@@ -148,6 +148,31 @@ tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int
                   int32_t *rows, uint64_t *count, int64_t *rows_written);
 
 /*
+ * Kernel launcher of a generated worker plugin (a motif-specialised
+ * instantiation of the library's kernel template, compiled into the plugin).
+ * mode 0: write the kernel's resident workgroups per CU for lds_bytes of
+ * dynamic LDS to *blocks; mode 1: launch it with grid x 64 threads, lds_bytes
+ * of LDS, on `stream` (a hipStream_t), passing *kernel_args (the library's
+ * launch descriptor) by value.  Returns 0 on success.
+ */
+typedef int32_t (*tm_launch_fn)(int32_t mode, const void *kernel_args, int32_t grid, int32_t lds_bytes,
+                                void *stream, int32_t *blocks, void *user);
+
+/*
+ * tm_count / tm_enum with the plugin's own kernel instead of the library's
+ * table-driven one; staging, scheduling state, timing and result handling
+ * are the library's.  Replaces TMotifMatchingGPUImpl of a generated worker
+ * (system/templates/Impl/top.snp:1-10).
+ */
+tm_status tm_count_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                          int32_t delta, int32_t beg, int32_t end, const tm_tuning *tune,
+                          tm_launch_fn launcher, void *user, uint64_t *count);
+tm_status tm_enum_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                         int32_t delta, int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune,
+                         tm_launch_fn launcher, void *user, int32_t *rows, uint64_t *count,
+                         int64_t *rows_written);
+
+/*
  * Count many motif queries on one graph in a single persistent launch.
  * Query q uses edges msrc/mdst[q * TM_MAX_MOTIF_EDGES ...], num_edges[q],
  * deltas[q] and roots [0, E - num_edges[q] + 1).  counts[q] receives the

@@ -1,0 +1,99 @@
+"""End to end on the GPU through the reference-shaped API: YAML query ->
+generated gfx950 worker plugin -> GPUWorkerLibManager -> SingleGPUExecution,
+and the stock plugin libTM.so as in the reference's legacy runner; counts and
+enumerated rows bit-exact against the oracle."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle, prep
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def inputs(tmp_path_factory):
+    from everest_amd.gen import graphs
+    root = tmp_path_factory.mktemp("inputs")
+    (root / "graphs").mkdir()
+    os.symlink(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "inputs", "motifs"),
+               root / "motifs")
+    U, V, T = graphs.make("zipf100k")
+    graphs.write_edge_file(str(root / "graphs" / "zipf.txt"), U, V, T)
+    os.environ["EVEREST_INPUTS"] = str(root)
+    return root, prep.preprocess(U, V, T)
+
+
+def _query(root, name, pattern, delta, extra=""):
+    p = root / name
+    p.write_text("pattern: %s\ngraph: zipf.txt\nconstraints:\n  delta: %d\noptions:\n  gpu: 1\n%s" % (
+        pattern, delta, extra))
+    return str(p)
+
+
+def test_generated_worker_counts(inputs):
+    from everest_amd.tmine import frontend
+    from oracle import motif as omotif
+    root, g = inputs
+    m = None
+    for pat, d in (("cycle3.txt", 3600), ("M1.txt", 1800), ("M4.txt", 7200), ("hub5.txt", 3600)):
+        q = _query(root, "q_%s.yaml" % pat, pat, d)
+        say("frontend query", pat, d)
+        if m is None:
+            m = frontend.MiningSingle(q)
+            cn, _, _ = m.run()
+        else:
+            cn, _, _ = m.run(q)
+        mot = omotif.read_motif(os.path.join(str(root), "motifs", pat))
+        assert cn == oracle.count(g, mot, d)
+
+
+def test_generated_worker_enumerates(inputs):
+    from everest_amd.tmine import frontend
+    from oracle import motif as omotif
+    root, g = inputs
+    q = _query(root, "qe.yaml", "cycle3.txt", 1200, "  enumerate: 100000\n")
+    say("frontend enumerate cycle3 1200")
+    cn, _, _, rows = frontend.MiningSingle(q).run()
+    mot = omotif.read_motif(os.path.join(str(root), "motifs", "cycle3.txt"))
+    total, want = oracle.enum(g, mot, 1200, 100000)
+    assert cn == total
+    assert sorted(map(tuple, rows.tolist())) == sorted(map(tuple, want.tolist()))
+
+
+def test_stock_plugin_and_executions(inputs):
+    """The reference's legacy flow (system/legacy.py) with libTM.so."""
+    from everest_amd import core
+    from everest_amd.abi import LIB_DIR
+    from oracle import motif as omotif
+    root, g = inputs
+    el = core.data.EdgeListLoader(str(root / "graphs" / "zipf.txt"))
+    hg = core.data.GraphDataLoader(el, core.data.FeatureLoader(el, "")).createGraphData()
+    motif_path = os.path.join(str(root), "motifs", "M3.txt")
+    hm = core.data.HostMotifData(motif_path)
+    dm = core.data.MakeGraphDataManager(1)
+    lm = core.GPUWorkerLibManager()
+    lm.openLib(os.path.join(LIB_DIR, "libTM.so"))
+    w = lm.construct("TM", 0)
+    ex = core.SingleGPUExecution(dm)
+    ex.setHostGraph(hg)
+    ex.setHostMotif(hm)
+    ex.setWorker(w)
+    say("legacy flow M3 delta 3600")
+    want = oracle.count(g, omotif.read_motif(motif_path), 3600)
+    assert ex.run(3600) == want
+    assert ex.run(3600) == 2 * want  # run() is cumulative over the worker's lifetime, as the reference's
+    div = core.SingleGPUExecutionDiv(dm, 0, 7)
+    div.setHostGraph(hg)
+    div.setHostMotif(hm)
+    w2 = lm.construct("TM", 0)
+    div.setWorker(w2)
+    assert div.run(3600) == want
+    del ex, div, w, w2

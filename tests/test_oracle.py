@@ -16,7 +16,7 @@ from helpers import random_graph, random_motif
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_READER = os.path.join(ROOT, "oracle", "_ref", "motif_ref")
-REF_MOTIFS = os.path.join(ROOT, "tests", "golden", "ref_motifs")
+REF_MOTIFS = os.path.join(ROOT, "inputs", "motifs")
 
 
 def test_oracle_matches_brute_force_counts_and_rows():
