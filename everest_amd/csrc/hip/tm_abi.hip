@@ -114,6 +114,27 @@ __global__ void tfirst_kernel(const tm_edge *__restrict__ Eg, const int *__restr
   t5[p] = (lo < outR[e.v + 1] && outPairNbr[lo] == e.u) ? T[outPairPos[lo]] : NONE;
 }
 
+// Root gaps: first-candidate time minus the root's own time, per rule.
+__global__ void rgap_kernel(const tm_edge *__restrict__ Eg, int E, const int *t0, const int *t1, const int *t2,
+                            const int *t3, const int *t4, const int *t5, unsigned *g0, unsigned *g1, unsigned *g2,
+                            unsigned *g3, unsigned *g4, unsigned *g5) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  const tm_edge e = Eg[p];
+  const int *tf[TM_NTFIRST] = {t0, t1, t2, t3, t4, t5};
+  unsigned *out[TM_NTFIRST] = {g0, g1, g2, g3, g4, g5};
+#pragma unroll
+  for (int k = 0; k < TM_NTFIRST; k++) {
+    const int f = tf[k][p];
+    unsigned gap = 0xffffffffu;
+    if (e.u != e.v && f != 0x7fffffff) {
+      const long long d = (long long)f - e.t;
+      gap = d > 0x7fffffffll ? 0x7fffffffu : (unsigned)d;
+    }
+    out[k][p] = gap;
+  }
+}
+
 __global__ void low_word_kernel(const unsigned long long *__restrict__ k, int *__restrict__ out, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (int)(unsigned)(k[i] & 0xffffffffull);
@@ -262,17 +283,52 @@ static tm_status dispatch(tm_graph *g, int nmap, bool enumerate, bool instr, int
   return enumerate ? launch_t<8, true, false>(g, *a, grid, lds) : launch_t<8, false, false>(g, *a, grid, lds);
 }
 
-// Plan and launch the mining kernel for nq queries staged in scratch.h_queries:
-// size the persistent grid from occupancy, pick the chunk size, lay the
-// queries' root chunks out in one index space, upload, reset, launch.
-tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const tm_tuning *tune, bool instr,
+// Plan and launch the mining kernel for staged queries [q0, q0 + nq) of
+// scratch.h_queries (nq <= TM_LAUNCH_MAXQ): size the persistent grid from
+// occupancy, pick the chunk size over the union of the root ranges, upload,
+// reset, launch.
+tm_status launch(tm_graph *g, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
                  tm_launch_fn fn = nullptr, void *user = nullptr) {
   Scratch &s = g->scratch;
+  int maxM = 2, maxN = 2, root_lo = INT32_MAX, root_hi = 0;
+  std::vector<int> order(nq);
+  for (int i = 0; i < nq; i++) {
+    const tm_query_dev &qd = s.h_queries[q0 + i];
+    if (qd.m.tmode[1] < 0 || qd.m.tmode[1] >= TM_NTFIRST) return fail(TM_ERR_INTERNAL, "edge 1 has no start rule");
+    maxM = std::max(maxM, qd.m.M);
+    maxN = std::max(maxN, qd.m.n);
+    root_lo = std::min(root_lo, qd.root_beg);
+    root_hi = std::max(root_hi, qd.root_end);
+    order[i] = i;
+  }
+  if (root_hi < root_lo) root_hi = root_lo;
+  // dispatch order: by start rule of edge 1, then by decreasing delta, so a
+  // wavefront stops scanning a rule's queries at the first delta below the
+  // smallest gap of its roots
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const tm_query_dev &a = s.h_queries[q0 + x], &b = s.h_queries[q0 + y];
+    if (a.m.tmode[1] != b.m.tmode[1]) return a.m.tmode[1] < b.m.tmode[1];
+    return a.delta > b.delta;
+  });
+  int rule_mask = 0, rule_first[TM_NTFIRST], rule_end[TM_NTFIRST], rule_maxdelta[TM_NTFIRST];
+  for (int k = 0; k < TM_NTFIRST; k++) { rule_first[k] = rule_end[k] = 0; rule_maxdelta[k] = -1; }
+  for (int i = 0; i < nq; i++) {
+    const tm_query_dev &qd = s.h_queries[q0 + order[i]];
+    const int k = qd.m.tmode[1];
+    if (!((rule_mask >> k) & 1)) { rule_mask |= 1 << k; rule_first[k] = i; rule_maxdelta[k] = qd.delta; }
+    rule_end[k] = i + 1;
+    int *qt = &s.h_qtab[4 * (size_t)(q0 + i)];
+    qt[0] = order[i];
+    qt[1] = qd.delta;
+    qt[2] = qd.root_beg;
+    qt[3] = qd.root_end;
+  }
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
   const int nmap = maxN <= 4 ? 4 : 8;
   const int per_frame = 6 + nmap + (enumerate ? TM_MAXE : 0);
-  const int lds = frames * per_frame * (int)sizeof(int);
+  // frame stack (an even number of ints), then the launch's per-query counts
+  const int lds = frames * per_frame * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
   int blocks = 0;
   tm_status st;
   if (fn) {
@@ -287,38 +343,41 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   if (wpc < 1) wpc = 1;
   int grid = g->num_cus * wpc;
 
-  long long roots = 0;
-  for (int q = 0; q < nq; q++) roots += (long long)(s.h_queries[q].root_end - s.h_queries[q].root_beg);
+  const int root_al = root_lo & ~3;
+  const long long roots = (long long)root_hi - root_al;
   int chunk_roots;
   if (tune && tune->chunk_roots > 0) {
-    chunk_roots = (tune->chunk_roots + 63) / 64 * 64;
+    chunk_roots = (tune->chunk_roots + 255) / 256 * 256;
   } else {
     // about 8 chunks per wavefront: few atomics, still fine-grained at the end
     long long c = (roots + (long long)grid * 8 - 1) / ((long long)grid * 8);
-    c = (c + 63) / 64 * 64;
-    chunk_roots = (int)std::min<long long>(std::max<long long>(c, 256), 8192);
+    c = (c + 255) / 256 * 256;
+    chunk_roots = (int)std::min<long long>(std::max<long long>(c, 1024), 16384);
   }
-  long long acc = 0;
-  for (int q = 0; q < nq; q++) {
-    s.h_prefix[q] = acc;
-    acc += ((long long)(s.h_queries[q].root_end - s.h_queries[q].root_beg) + chunk_roots - 1) / chunk_roots;
-  }
-  s.h_prefix[nq] = acc;
-  const long long total_chunks = acc;
+  const long long total_chunks = (roots + chunk_roots - 1) / chunk_roots;
   if (total_chunks < grid) grid = (int)std::max<long long>(1, total_chunks);
 
-  HIPTRY(hipMemcpyAsync(s.d_queries, s.h_queries.data(), sizeof(tm_query_dev) * nq, hipMemcpyHostToDevice,
-                        g->stream));
-  HIPTRY(hipMemcpyAsync(s.d_prefix, s.h_prefix.data(), sizeof(long long) * (nq + 1), hipMemcpyHostToDevice,
-                        g->stream));
+  HIPTRY(hipMemcpyAsync(s.d_queries + q0, s.h_queries.data() + q0, sizeof(tm_query_dev) * nq,
+                        hipMemcpyHostToDevice, g->stream));
+  HIPTRY(hipMemcpyAsync(s.d_qtab + 4 * (size_t)q0, s.h_qtab.data() + 4 * (size_t)q0, sizeof(int) * 4 * nq,
+                        hipMemcpyHostToDevice, g->stream));
 
   tm_kernel_args a;
   memset(&a, 0, sizeof(a));
   a.g = g->view;
-  a.queries = s.d_queries;
+  a.queries = s.d_queries + q0;
+  a.qtab = s.d_qtab + 4 * (size_t)q0;
   a.nq = nq;
   a.chunk_roots = chunk_roots;
-  a.chunk_prefix = s.d_prefix;
+  a.root_lo = root_lo;
+  a.root_hi = root_hi;
+  a.root_al = root_al;
+  a.rule_mask = rule_mask;
+  for (int k = 0; k < TM_NTFIRST; k++) {
+    a.rule_first[k] = rule_first[k];
+    a.rule_end[k] = rule_end[k];
+    a.rule_maxdelta[k] = rule_maxdelta[k];
+  }
   a.total_chunks = total_chunks;
   a.frames = frames;
   a.scan_max = (tune && tune->scan_max > 0) ? tune->scan_max : 16;
@@ -340,7 +399,7 @@ tm_status launch(tm_graph *g, int nq, int maxM, int maxN, bool enumerate, const 
   g->last_grid = grid;
 
   hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, g->stream, s.d_sched,
-                     s.d_counts, s.d_stops, nq);
+                     s.d_counts + q0, s.d_stops + q0, nq);
   HIPTRY(hipGetLastError());
   if (fn) {
     HIPTRY(hipEventRecord(g->ev0, g->stream));
@@ -360,8 +419,8 @@ tm_status reserve(tm_graph *g, int nq) {
   tm_status st;
   if ((st = dev_alloc(g, &p, sizeof(tm_query_dev) * cap)) != TM_OK) return st;
   s.d_queries = (tm_query_dev *)p;
-  if ((st = dev_alloc(g, &p, sizeof(long long) * (cap + 1))) != TM_OK) return st;
-  s.d_prefix = (long long *)p;
+  if ((st = dev_alloc(g, &p, sizeof(int) * 4 * cap)) != TM_OK) return st;
+  s.d_qtab = (int *)p;
   if ((st = dev_alloc(g, &p, sizeof(unsigned long long) * cap)) != TM_OK) return st;
   s.d_counts = (unsigned long long *)p;
   if ((st = dev_alloc(g, &p, sizeof(int) * cap)) != TM_OK) return st;
@@ -378,7 +437,7 @@ tm_status reserve(tm_graph *g, int nq) {
   }
   s.cap_q = cap;
   s.h_queries.resize(cap);
-  s.h_prefix.resize(cap + 1);
+  s.h_qtab.resize(4 * (size_t)cap);
   return TM_OK;
 }
 
@@ -485,6 +544,10 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   for (int k = 0; k < TM_NTFIRST; k++) {
     ALLOC(vw.tfirst[k], e4);
   }
+  vw.epad = (int)(((long long)E + 255) / 256 * 256 + 256);
+  for (int k = 0; k < TM_NTFIRST; k++) {
+    ALLOC(vw.rgap[k], sizeof(unsigned) * (size_t)vw.epad);
+  }
   const int tmin = Eg[0].t, tmax = Eg[E - 1].t;
   const long long span = (long long)tmax - tmin + 1;
   int bshift = ceil_log2(std::max(1ll, (span * 16) / std::max(1, E)));
@@ -549,6 +612,13 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
                        vw.outPairSelf, (int *)vw.tfirst[0], (int *)vw.tfirst[1], (int *)vw.tfirst[2],
                        (int *)vw.tfirst[3], (int *)vw.tfirst[4], (int *)vw.tfirst[5]);
     TMPCHK(hipGetLastError());
+    for (int k = 0; k < TM_NTFIRST; k++)
+      TMPCHK(hipMemsetAsync((void *)vw.rgap[k], 0xff, sizeof(unsigned) * (size_t)vw.epad, s));
+    hipLaunchKernelGGL(rgap_kernel, gr, bl, 0, s, vw.Eg, E, vw.tfirst[0], vw.tfirst[1], vw.tfirst[2], vw.tfirst[3],
+                       vw.tfirst[4], vw.tfirst[5], (unsigned *)vw.rgap[0], (unsigned *)vw.rgap[1],
+                       (unsigned *)vw.rgap[2], (unsigned *)vw.rgap[3], (unsigned *)vw.rgap[4],
+                       (unsigned *)vw.rgap[5]);
+    TMPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((nb + 1 + 255) / 256)), dim3(256), 0, s, vw.T, E, tmin,
                      bshift, nb, (int *)vw.bucket);
@@ -597,12 +667,10 @@ tm_status tm_graph_info(const tm_graph *g, int64_t *device_bytes, int32_t *num_e
 
 static tm_status stage_queries(tm_graph *g, int nq, const int32_t *num_edges, const int32_t *msrc,
                                const int32_t *mdst, int stride, const int32_t *deltas,
-                               const int32_t *begs, const int32_t *ends, int *maxM, int *maxN) {
+                               const int32_t *begs, const int32_t *ends) {
   tm_status st;
   if ((st = reserve(g, nq)) != TM_OK) return st;
   Scratch &s = g->scratch;
-  *maxM = 2;
-  *maxN = 2;
   for (int q = 0; q < nq; q++) {
     tm_query_dev &qd = s.h_queries[q];
     memset(&qd, 0, sizeof(qd));
@@ -620,17 +688,15 @@ static tm_status stage_queries(tm_graph *g, int nq, const int32_t *num_edges, co
     qd.root_end = end;
     qd.count = s.d_counts + q;
     qd.stop = s.d_stops + q;
-    *maxM = std::max(*maxM, qd.m.M);
-    *maxN = std::max(*maxN, qd.m.n);
   }
   return TM_OK;
 }
 
-static tm_status finish(tm_graph *g, int nq, uint64_t *counts) {
+static tm_status finish(tm_graph *g, int q0, int nq, uint64_t *counts) {
   Scratch &s = g->scratch;
   tm_sched hs;
   HIPTRY(hipMemcpyAsync(&hs, s.d_sched, sizeof(hs), hipMemcpyDeviceToHost, g->stream));
-  HIPTRY(hipMemcpyAsync(counts, s.d_counts, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, g->stream));
+  HIPTRY(hipMemcpyAsync(counts, s.d_counts + q0, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, g->stream));
   HIPTRY(hipStreamSynchronize(g->stream));
   memcpy(g->last_stats, hs.stats, sizeof(g->last_stats));
   float ms = 0.f;
@@ -646,11 +712,10 @@ static tm_status count_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc,
   if (!g || !msrc || !mdst || !count) return fail(TM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  int maxM, maxN;
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, &maxM, &maxN);
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
-  if ((st = launch(g, 1, maxM, maxN, false, tune, false, fn, user)) != TM_OK) return st;
-  return finish(g, 1, count);
+  if ((st = launch(g, 0, 1, false, tune, false, fn, user)) != TM_OK) return st;
+  return finish(g, 0, 1, count);
 }
 
 tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
@@ -670,11 +735,10 @@ tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, con
   if (!g || !msrc || !mdst || !count || !work) return fail(TM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  int maxM, maxN;
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, &maxM, &maxN);
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
-  if ((st = launch(g, 1, maxM, maxN, false, tune, true)) != TM_OK) return st;
-  if ((st = finish(g, 1, count)) != TM_OK) return st;
+  if ((st = launch(g, 0, 1, false, tune, true)) != TM_OK) return st;
+  if ((st = finish(g, 0, 1, count)) != TM_OK) return st;
   const tm_query_dev &q = g->scratch.h_queries[0];
   work[0] = (uint64_t)(q.root_end - q.root_beg);
   work[1] = g->last_stats[TM_STAT_REF_X1];
@@ -688,12 +752,14 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
   if (!g || nq < 1 || !num_edges || !msrc || !mdst || !deltas || !counts) return fail(TM_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  int maxM, maxN;
-  tm_status st = stage_queries(g, nq, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas, nullptr, nullptr, &maxM,
-                               &maxN);
+  tm_status st = stage_queries(g, nq, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas, nullptr, nullptr);
   if (st != TM_OK) return st;
-  if ((st = launch(g, nq, maxM, maxN, false, tune, false)) != TM_OK) return st;
-  return finish(g, nq, counts);
+  for (int q0 = 0; q0 < nq; q0 += TM_LAUNCH_MAXQ) {
+    const int n = std::min(TM_LAUNCH_MAXQ, nq - q0);
+    if ((st = launch(g, q0, n, false, tune, false)) != TM_OK) return st;
+    if ((st = finish(g, q0, n, counts + q0)) != TM_OK) return st;
+  }
+  return TM_OK;
 }
 
 static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
@@ -719,8 +785,7 @@ static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, 
   if (!g || !msrc || !mdst || !count || cap < 0 || (cap > 0 && !rows)) return fail(TM_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  int maxM, maxN;
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end, &maxM, &maxN);
+  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
   Scratch &s = g->scratch;
   int32_t *d_rows = nullptr;
@@ -728,8 +793,8 @@ static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, 
   HIPTRY(hipMalloc(&d_rows, row_bytes));
   s.h_queries[0].rows = d_rows;
   s.h_queries[0].cap = cap;
-  if ((st = launch(g, 1, maxM, maxN, true, tune, false, fn, user)) != TM_OK) { (void)hipFree(d_rows); return st; }
-  st = finish(g, 1, count);
+  if ((st = launch(g, 0, 1, true, tune, false, fn, user)) != TM_OK) { (void)hipFree(d_rows); return st; }
+  st = finish(g, 0, 1, count);
   if (st == TM_OK) {
     int64_t n = std::min<int64_t>((int64_t)*count, cap);
     if (n > 0) {

@@ -6,13 +6,15 @@
 #include "tm_abi.h"
 #include "tm_types.h"
 
+#define TM_LAUNCH_MAXQ 512  // queries per launch (their table and counts live in LDS)
+
 namespace tmh {
 
 // Per-graph staging area for query descriptors and results (device + host mirrors).
 struct Scratch {
   int cap_q = 0;
   tm_query_dev *d_queries = nullptr;
-  long long *d_prefix = nullptr;
+  int *d_qtab = nullptr;                 // per query: first-candidate rule of edge 1, delta, root range
   unsigned long long *d_counts = nullptr;
   int *d_stops = nullptr;
   tm_sched *d_sched = nullptr;
@@ -20,7 +22,7 @@ struct Scratch {
   int pool_cap = 0;
   int gen = 0;                           // launch generation tag of pool words
   std::vector<tm_query_dev> h_queries;
-  std::vector<long long> h_prefix;
+  std::vector<int> h_qtab;
 };
 
 tm_status set_error(tm_status s, const std::string &msg);

@@ -27,6 +27,7 @@
 namespace tmk {
 
 constexpr int WAVE = 64;
+constexpr int TM_BLOCK = 4 * WAVE;  // roots per dispatch block (4 per lane)
 
 // ------------------------------------------------------------------ helpers
 
@@ -46,18 +47,27 @@ __device__ __forceinline__ int upper_bound_i(const int *__restrict__ a, int lo, 
   return lo;
 }
 
+// Register-array access by a run-time index.  The optimiser would turn a
+// select chain over array elements back into an indexed load from private
+// (scratch) memory; passing each element through an empty asm makes it an
+// opaque register value, so the array stays in VGPRs.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 template <int MAXN>
 __device__ __forceinline__ int sel(const int (&r)[MAXN], int idx) {
-  int v = r[0];
+  int v = opaque(r[0]);
 #pragma unroll
-  for (int i = 1; i < MAXN; i++) v = (i == idx) ? r[i] : v;
+  for (int i = 1; i < MAXN; i++) v = (i == idx) ? opaque(r[i]) : v;
   return v;
 }
 
 template <int MAXN>
 __device__ __forceinline__ void put(int (&r)[MAXN], int idx, int v) {
 #pragma unroll
-  for (int i = 0; i < MAXN; i++) r[i] = (i == idx) ? v : r[i];
+  for (int i = 0; i < MAXN; i++) r[i] = (i == idx) ? v : opaque(r[i]);
 }
 
 template <int MAXN>
@@ -66,6 +76,12 @@ __device__ __forceinline__ bool in_first(const int (&r)[MAXN], int n, int node) 
 #pragma unroll
   for (int i = 0; i < MAXN; i++) hit |= (i < n) && (r[i] == node);
   return hit;
+}
+
+__device__ __forceinline__ unsigned wave_min_u32(unsigned x) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) x = min(x, (unsigned)__shfl_xor((int)x, o));
+  return x;
 }
 
 __device__ __forceinline__ int wave_incl_scan(int x, int lane) {
@@ -326,21 +342,31 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   int *f_tl = f_lvl + FR;
   int *f_m = f_tl + FR;
   int *f_e = f_m + MAXN * FR;
+  // per-query counts of this wavefront, flushed once at the end
+  unsigned long long *q_cnt = (unsigned long long *)(f_e + (ENUM ? MAXE * FR : 0));
+  for (int q = lane; q < args.nq; q += WAVE) q_cnt[q] = 0;
+  __syncthreads();
 
   const tm_graph_view g = args.g;
   tm_sched *sched = args.sched;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   unsigned long long cnt = 0;
-  unsigned long long st_steps = 0, st_items = 0, st_roots = 0, st_leaves = 0, st_push = 0;
+  unsigned long long st_steps = 0, st_items = 0, st_roots = 0, st_push = 0;  // uniform
+  unsigned st_leaves = 0;  // per lane
   unsigned long long rx1 = 0, rx2 = 0, rp = 0;  // reference work (INSTR only)
   unsigned long long st_don = 0, st_take = 0, st_capexit = 0, st_polls = 0;
   unsigned iter = 0;
   int qcur = -1;
   int top = 0;
-  int chunk_left = 0;      // 64-root groups left in the current chunk
-  int next_root = 0;       // first root of the next 64-root group (uniform)
-  int root_end = 0;
+  int chunk_left = 0;      // 256-root blocks left in the current chunk
+  int next_root = 0;       // first root of the next block (uniform)
+  int blk = -1;            // first root of the block being dispatched (-1: none)
+  int quarter = 0;         // quarter of the block: this lane's root is blk + 4 lane + quarter
+  int rk = -1;             // start rule whose queries are being checked
+  int ent = 0, ent_end = 0;// next / end qtab entry of that rule
+  unsigned smin = 0;       // smallest gap of the quarter's roots under rule rk (uniform)
+  unsigned lgap = 0;       // this lane's gap under rule rk
   bool roots_done = false; // the root chunks of every query are handed out
   const int home_stripe = blockIdx.x & (TM_STRIPES - 1);
   int stripe_off = 0;
@@ -372,6 +398,13 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     }
     if (top == 0 && roots_done) {
       // ------------------------------------------ idle: take donated work
+      // Ticket protocol: an idle wavefront draws ticket t (one atomic on
+      // pool_head) and waits for record t.  Donors fill records in order
+      // (pool_tail) while tickets outnumber records.  sched->busy counts the
+      // wavefronts holding work plus the records reserved and not yet
+      // taken (a donor adds one before it reserves; the taker inherits it),
+      // so busy == 0 means no work is left anywhere and a waiting
+      // wavefront may leave.
       if (!is_idle) {
         int nidle = 0;
         if (lane == 0) { add_rlx(&sched->busy, -1); nidle = add_rlx(&sched->idle, 1) + 1; }
@@ -379,58 +412,40 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         // enough wavefronts are already waiting for donations: leave
         if (uni(__shfl(nidle, 0)) > args.idle_cap) { st_capexit++; break; }
       }
-      int slot = -1;
-      for (int spin = 0; spin < (1 << 20); spin++) {
-        st_polls++;
-        int h = 0, t = 0;
-        if (lane == 0) { h = (int)ld_rlx(&sched->pool_head); t = (int)ld_rlx(&sched->pool_tail); }
-        h = uni(__shfl(h, 0));
-        t = uni(__shfl(t, 0));
-        if (h < t && h < args.pool_cap) {
-          int won = 0;
-          if (lane == 0) {
-            add_rlx(&sched->busy, 1);
-            won = atomicCAS(&sched->pool_head, (unsigned)h, (unsigned)(h + 1)) == (unsigned)h;
-            if (!won) add_rlx(&sched->busy, -1);
-          }
-          if (uni(__shfl(won, 0))) { slot = h; break; }
-          __builtin_amdgcn_s_sleep(16);  // lost the race: back off before polling again
-          continue;
-        }
-        int b = 0, ab = 0;
-        if (lane == 0) { b = ld_rlx(&sched->busy); ab = ld_rlx(&sched->abort); }
-        if (uni(__shfl(ab, 0))) break;
-        if (uni(__shfl(b, 0)) == 0) {
-          // nobody holds work: anything reserved is already in the pool
-          if (lane == 0) { h = (int)ld_rlx(&sched->pool_head); t = (int)ld_rlx(&sched->pool_tail); }
-          h = uni(__shfl(h, 0));
-          t = uni(__shfl(t, 0));
-          if (h < t && h < args.pool_cap) continue;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(127);
-      }
-      if (slot < 0) break;
-      st_take++;
-      if (lane == 0) add_rlx(&sched->idle, -1);
-      is_idle = false;
-      // read the record until every word carries this launch's generation
-      const unsigned long long *rec = args.pool + (size_t)slot * TM_POOL_FIELDS;
+      int ticket = 0;
+      if (lane == 0) ticket = (int)atomicAdd(&sched->pool_head, 1u);
+      ticket = uni(__shfl(ticket, 0));
+      if (ticket >= args.pool_cap) break;  // no record is ever written there
+      const unsigned long long *rec = args.pool + (size_t)ticket * TM_POOL_FIELDS;
       int fv = 0;
-      bool complete = false;
-      for (int w = 0; w < (1 << 22); w++) {
+      bool got = false;
+      for (unsigned spin = 0;; spin++) {
+        st_polls++;
         unsigned long long word = lane < TM_POOL_FIELDS ? ld_rlx(rec + lane) : 0ull;
         const bool mine = lane >= TM_POOL_FIELDS || (int)(word >> 32) == args.gen;
         fv = (int)(unsigned)(word & 0xffffffffull);
-        if (__ballot(!mine) == 0ull) { complete = true; break; }
-        __builtin_amdgcn_s_sleep(2);
+        if (__ballot(!mine) == 0ull) { got = true; break; }
+        if ((spin & 3u) == 3u) {
+          int b = 0, ab = 0;
+          if (lane == 0) {
+            b = ld_rlx(&sched->busy);
+            ab = ld_rlx(&sched->abort);
+            if (args.limit_ticks > 0 && (long long)wall_clock64() - sched->t0 > args.limit_ticks) ab = 1;
+          }
+          if (uni(__shfl(ab, 0))) break;
+          if (uni(__shfl(b, 0)) == 0) break;  // every record is taken and nobody holds work
+        }
+        __builtin_amdgcn_s_sleep(32);
       }
-      if (!complete) { if (lane == 0) atomicOr(&sched->error, 8); break; }
+      if (!got) break;
+      st_take++;
+      if (lane == 0) add_rlx(&sched->idle, -1);
+      is_idle = false;  // the record's busy credit is now this wavefront's
       const int fq = uni(__shfl(fv, 0));
       if (fq != qcur) {
         if (!ENUM && qcur >= 0) {
           unsigned long long sm = wave_sum64(cnt);
-          if (lane == 0 && sm) atomicAdd(args.queries[qcur].count, sm);
+          if (lane == 0 && sm) q_cnt[qcur] += sm;
         }
         cnt = 0;
         qcur = fq;
@@ -446,76 +461,127 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     }
     if (top == 0) {
       // ---------------------------------------------------- root dispatch
-      if (chunk_left == 0) {
-        // striped counters: stripe k hands out chunks [k C / S, (k+1) C / S);
-        // a wavefront starts on its own stripe and moves on when it runs dry
-        long long c = -1;
-        while (stripe_off < TM_STRIPES) {
-          const int k = (home_stripe + stripe_off) & (TM_STRIPES - 1);
-          const long long sb = args.total_chunks * k / TM_STRIPES;
-          const long long se = args.total_chunks * (k + 1) / TM_STRIPES;
-          long long o = 0;
-          if (lane == 0) o = (long long)atomicAdd(&sched->next_chunk[k][0], 1ull);
-          o = __shfl(o, 0);
-          if (sb + o < se) { c = sb + o; break; }
-          stripe_off++;
-        }
-        if (c < 0) { roots_done = true; continue; }
-        // locate the query owning chunk c (uniform binary search)
-        int lo = 0, hi = args.nq;
-        while (hi - lo > 1) {
-          int mid = (lo + hi) >> 1;
-          if (args.chunk_prefix[mid] <= c) lo = mid; else hi = mid;
-        }
-        const int qi = uni(lo);
-        if (qi != qcur) {
-          if (!ENUM && qcur >= 0) {
-            unsigned long long s = wave_sum64(cnt);
-            if (lane == 0 && s) atomicAdd(args.queries[qcur].count, s);
+      // Roots are scanned in 256-root blocks over the union of the queries'
+      // root ranges; lane l holds roots blk + 4l .. blk + 4l + 3 and their
+      // gaps (one 16-byte load per start rule in use).  A block none of
+      // whose gaps is within the largest delta of its rule is skipped whole.
+      // Otherwise each quarter of the block (one root per lane) is checked
+      // against the queries of each rule in decreasing delta order, stopping
+      // at the first delta below the quarter's smallest gap, and the
+      // wavefront expands every query some lane passes, one at a time.
+      int qsel = -1;
+      bool pass = false, inq = false;
+      while (qsel < 0) {
+        if (blk < 0) {
+          if (chunk_left == 0) {
+            // striped counters: stripe k hands out chunks [k C / S, (k+1) C / S);
+            // a wavefront starts on its own stripe and moves on when it runs dry
+            long long c = -1;
+            while (stripe_off < TM_STRIPES) {
+              const int k = (home_stripe + stripe_off) & (TM_STRIPES - 1);
+              const long long sb = args.total_chunks * k / TM_STRIPES;
+              const long long se = args.total_chunks * (k + 1) / TM_STRIPES;
+              long long o = 0;
+              if (lane == 0) o = (long long)atomicAdd(&sched->next_chunk[k][0], 1ull);
+              o = __shfl(o, 0);
+              if (sb + o < se) { c = sb + o; break; }
+              stripe_off++;
+            }
+            if (c < 0) { roots_done = true; break; }
+            const long long first = (long long)args.root_al + c * (long long)args.chunk_roots;
+            next_root = (int)first;
+            const long long left = ((long long)args.root_hi - first + TM_BLOCK - 1) / TM_BLOCK;
+            const int cap = args.chunk_roots / TM_BLOCK;
+            chunk_left = (int)(left < cap ? left : cap);
+            if (chunk_left <= 0) { chunk_left = 0; continue; }
           }
-          cnt = 0;
-          qcur = qi;
+          blk = next_root;
+          next_root += TM_BLOCK;
+          chunk_left--;
+          const int r0 = blk + 4 * lane;
+          bool near = false;
+#pragma unroll
+          for (int k = 0; k < TM_NTFIRST; k++) {
+            if ((args.rule_mask >> k) & 1) {
+              uint4 x = *(const uint4 *)(g.rgap[k] + r0);
+              if (r0 < args.root_lo || r0 >= args.root_hi) x.x = ~0u;
+              if (r0 + 1 < args.root_lo || r0 + 1 >= args.root_hi) x.y = ~0u;
+              if (r0 + 2 < args.root_lo || r0 + 2 >= args.root_hi) x.z = ~0u;
+              if (r0 + 3 < args.root_lo || r0 + 3 >= args.root_hi) x.w = ~0u;
+              const unsigned mn = min(min(x.x, x.y), min(x.z, x.w));
+              near |= args.rule_maxdelta[k] >= 0 && mn <= (unsigned)args.rule_maxdelta[k];
+            }
+          }
+          st_roots += (unsigned)(min(max(args.root_hi - blk, 0), TM_BLOCK) - min(max(args.root_lo - blk, 0), TM_BLOCK));
+          if (!INSTR && __ballot(near) == 0ull) { blk = -1; continue; }
+          quarter = 0;
+          rk = -1;
+          ent = ent_end = 0;
         }
-        const tm_query_dev &q = args.queries[qcur];
-        if (ENUM && ld_rlx(q.stop)) continue;
-        long long first = (long long)q.root_beg + (c - args.chunk_prefix[qcur]) * (long long)args.chunk_roots;
-        next_root = (int)first;
-        root_end = q.root_end;
-        long long left = ((long long)root_end - first + WAVE - 1) / WAVE;
-        int cap = args.chunk_roots / WAVE;
-        chunk_left = (int)(left < cap ? left : cap);
-        if (chunk_left <= 0) { chunk_left = 0; continue; }
+        if (ent >= ent_end) {
+          // next rule in use for this quarter, else the next quarter
+          int k = rk + 1;
+          while (k < TM_NTFIRST && !((args.rule_mask >> k) & 1)) k++;
+          if (k >= TM_NTFIRST) {
+            if (++quarter == 4) { blk = -1; continue; }
+            k = 0;
+            while (!((args.rule_mask >> k) & 1)) k++;
+          }
+          rk = k;
+          ent = args.rule_first[k];
+          ent_end = args.rule_end[k];
+          {
+            // the block was just read: this re-read of one quarter hits in cache
+            const int r = blk + 4 * lane + quarter;
+            lgap = (r >= args.root_lo && r < args.root_hi) ? g.rgap[rk][r] : ~0u;
+          }
+          smin = uni(wave_min_u32(lgap));
+          continue;
+        }
+        const int4 qt = *(const int4 *)(args.qtab + 4 * ent);  // (query, delta, root_beg, root_end)
+        if (!INSTR && (qt.y < 0 || smin > (unsigned)qt.y)) { ent = ent_end; continue; }
+        ent++;
+        if (ENUM && ld_rlx(args.queries[qt.x].stop)) continue;
+        const int r = blk + 4 * lane + quarter;
+        inq = r >= args.root_lo && r < args.root_hi && r >= qt.z && r < qt.w;
+        pass = inq && qt.y >= 0 && lgap <= (unsigned)qt.y;
+        if (__ballot(pass) != 0ull || (INSTR && __ballot(inq) != 0ull)) qsel = qt.x;
+      }
+      if (qsel < 0) continue;  // every root chunk is handed out
+      if (qsel != qcur) {
+        if (!ENUM && qcur >= 0) {
+          unsigned long long sm = wave_sum64(cnt);
+          if (lane == 0 && sm) q_cnt[qcur] += sm;
+        }
+        cnt = 0;
+        qcur = qsel;
       }
       const tm_query_dev &q = args.queries[qcur];
       const tm_motif_dev &md = q.m;
       const int M = SP::M(md);
-      const int r = next_root + lane;
-      next_root += WAVE;
-      chunk_left--;
-      st_roots += (r < root_end) ? 1 : 0;
-      bool ok = r < root_end;
+      const int r = blk + 4 * lane + quarter;
       int u = 0, v = 0, t = 0;
-      if (ok) {
+      bool gvalid = false;
+      if (pass || (INSTR && inq)) {
         const tm_edge e = g.Eg[r];
         u = e.u; v = e.v; t = e.t;
-        ok = (u != v);
+        gvalid = u != v;
       }
+      bool ok = pass;
       int m[MAXN];
 #pragma unroll
       for (int i = 0; i < MAXN; i++) m[i] = 0;
       m[0] = u; m[1] = v;
-      // candidates of edge 1; most roots stop at the first candidate's time
       CandList L{g.outPos, g.outNbr, 0, 0, 0};
       const long long tl = (long long)t + q.delta;
       const int tli = tl > 0x7fffffffll ? 0x7fffffff : (int)tl;
-      if (ok && SP::tmode(md, 1) >= 0) ok = g.tfirst[SP::tmode(md, 1)][r] <= tli;  // streamed, coalesced
       if (ok) {
         L = cand_start<SP>(g, md, 1, m, r);
         ok = L.lo < L.hi && (long long)g.T[L.pos[L.lo]] <= tl;
       }
       int pend = 0;
       if constexpr (INSTR) {
-        if (r < root_end && u != v) {
+        if (gvalid) {
           const int pe = pend_of(g, tl);
           (SP::kind(md, 1) == TM_K_CLOSE ? rx2 : rx1) += 1;
           rp += ref_probes<SP>(g, md, 1, m, r, pe);
@@ -591,7 +657,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     const bool has = lane < total;
     const int myf = top - 1 - rank;
     st_steps++;
-    st_items += has ? 1 : 0;
+    st_items += total;
     int slot = 0, p = 0, pend = 0, dir = 0, tl = 0;
     int m[MAXN];
     int e[ENUM ? MAXE : 1];
@@ -693,7 +759,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         f_e[j * FR + w] = pos;
       }
     }
-    st_push += valid ? 1 : 0;
+    st_push += n;
     top = ntop + n;
     __syncthreads();
     // ---------------------------------------------- donate to idle wavefronts
@@ -701,14 +767,12 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     // donates: short tasks never pay for the exchange
     if (top > 0 && (st_steps % (unsigned long long)args.steal_every) == 0 &&
         (long long)wall_clock64() - task_t0 > args.steal_after_ticks) {
-      // donate only against unmet demand: waiting wavefronts minus frames
-      // already waiting in the pool
+      // donate only against unmet demand: tickets drawn minus records reserved
       int want = 0;
       if (lane == 0) {
-        const int idl = ld_rlx(&sched->idle);
         const int hd = (int)ld_rlx(&sched->pool_head);
-        const int tl = min((int)ld_rlx(&sched->pool_tail), args.pool_cap);
-        want = min(idl - (tl - hd), args.steal_max);
+        const int tl = (int)ld_rlx(&sched->pool_tail);
+        want = min(min(hd, args.pool_cap) - tl, args.steal_max);
       }
       want = uni(__shfl(want, 0));
       for (int d = 0; d < want; d++) {
@@ -718,7 +782,11 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         if (cm == 0ull) break;
         const int b = __ffsll((long long)cm) - 1;
         int t = 0;
-        if (lane == 0) t = (int)atomicAdd(&sched->pool_tail, 1u);
+        if (lane == 0) {
+          add_rlx(&sched->busy, 1);  // the record's credit, before it can be seen
+          t = (int)atomicAdd(&sched->pool_tail, 1u);
+          if (t >= args.pool_cap) add_rlx(&sched->busy, -1);
+        }
         t = uni(__shfl(t, 0));
         if (t >= args.pool_cap) break;
         const int cur0 = f_cur[b], end0 = f_end[b];
@@ -744,15 +812,18 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
 
   if (!ENUM && qcur >= 0) {
     unsigned long long s = wave_sum64(cnt);
-    if (lane == 0 && s) atomicAdd(args.queries[qcur].count, s);
+    if (lane == 0 && s) q_cnt[qcur] += s;
   }
+  __syncthreads();
+  if (!ENUM)
+    for (int q = lane; q < args.nq; q += WAVE)
+      if (q_cnt[q]) atomicAdd(args.queries[q].count, q_cnt[q]);
   if (lane == 0) {
     if (is_idle) add_rlx(&sched->idle, -1);
     else add_rlx(&sched->busy, -1);
   }
   // work counters: one atomic per counter per wavefront
-  unsigned long long w0 = wave_sum64(st_items), w1 = wave_sum64(st_roots), w2 = wave_sum64(st_leaves),
-                     w3 = wave_sum64(st_push);
+  unsigned long long w0 = st_items, w1 = st_roots, w2 = wave_sum64(st_leaves), w3 = st_push;
   if (lane == 0) {
     atomicAdd(&sched->stats[TM_STAT_STEPS], st_steps);
     atomicAdd(&sched->stats[TM_STAT_ITEMS], w0);

@@ -67,6 +67,13 @@ typedef struct {
   const int *inNext;      // first slot of in(src of p) with position > p
   const int *outPairSelf; // slot of position p in the out-pair index of src of p
   const int *tfirst[TM_NTFIRST];  // TM_T_* arrays, one entry per position
+  // Root gaps, one array per TM_T_* rule: tfirst - T[p] clamped to INT_MAX,
+  // or 0xffffffff when p is a self-loop or has no candidate; a root passes a
+  // query with that rule for edge 1 iff gap <= delta.  Padded with
+  // 0xffffffff to a multiple of 256 entries plus 256, so a 256-root block
+  // read as one 16-byte load per lane never leaves the array.
+  const unsigned *rgap[TM_NTFIRST];
+  int epad;               // entries per rgap array
   const int *bucket;      // bucket[k] = first position with T >= tmin + (k << bshift)
   int tmin, bshift;
   long long nbuckets;
@@ -93,10 +100,16 @@ typedef struct {
   int abort;        // set once the deadline passed: every wavefront stops
   long long t0;     // wall clock at launch (set by the reset kernel)
   unsigned long long stats[16];  // work counters (see TM_STAT_*)
-  unsigned int pool_head;  // next donated frame to take
-  unsigned int pool_tail;  // donated frames reserved so far
-  int busy;         // wavefronts holding work (may still donate)
+  // each on its own 128-byte line: donors, takers and waiters do not queue
+  // behind each other's atomics
+  unsigned int pool_head;  // tickets drawn by idle wavefronts
+  int pad0[31];
+  unsigned int pool_tail;  // donated records reserved so far
+  int pad1[31];
+  int busy;         // wavefronts holding work + records reserved and not yet taken
+  int pad2[31];
   int idle;         // wavefronts waiting for donated work
+  int pad3[31];
 } tm_sched;
 
 // A frame donated by a busy wavefront to an idle one (global pool): fields
@@ -124,9 +137,17 @@ enum {
 typedef struct {
   tm_graph_view g;
   const tm_query_dev *queries;
+  // Query table in dispatch order: sorted by the start rule of edge 1, then
+  // by decreasing delta; 4 ints per entry: (query, delta, root_beg, root_end).
+  const int *qtab;
   int nq;
-  int chunk_roots;            // roots per dispatched chunk (multiple of 64)
-  const long long *chunk_prefix;  // nq + 1 entries: first chunk of each query
+  int chunk_roots;            // roots per dispatched chunk (multiple of 256)
+  int root_lo, root_hi;       // union of the queries' root ranges
+  int root_al;                // root_lo rounded down to a multiple of 4 (first block)
+  int rule_mask;              // start rules used by some query
+  int rule_first[TM_NTFIRST]; // qtab entries [rule_first[k], rule_end[k]) have rule k
+  int rule_end[TM_NTFIRST];
+  int rule_maxdelta[TM_NTFIRST]; // largest delta of rule k (-1: no query)
   long long total_chunks;
   int frames;                 // LDS frame capacity per wavefront
   int scan_max;               // last-edge ranges up to this size are scanned

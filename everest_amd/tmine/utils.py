@@ -51,6 +51,19 @@ def compileModule(file, out):
     return out
 
 
+def headers_digest():
+    """Hash of every header a worker includes: a kernel or ABI change makes
+    cached plugins stale even when the generated source is the same."""
+    h = hashlib.sha256()
+    for d in locations.includeDirs():
+        for root, _, files in sorted(os.walk(d)):
+            for f in sorted(files):
+                if f.endswith((".h", ".cuh", ".hpp")):
+                    with open(os.path.join(root, f), "rb") as fh:
+                        h.update(f.encode() + fh.read())
+    return h.hexdigest()
+
+
 def compilePlugin(source=None, out=None):
     """Compile the generated worker (cached by source hash) into libDYN.so."""
     source = source or locations.dynLibSource()
@@ -58,7 +71,7 @@ def compilePlugin(source=None, out=None):
     createDirIfNotExist(os.path.dirname(out))
     createDirIfNotExist(locations.pluginCacheRoot())
     with open(source, "rb") as f:
-        key = hashlib.sha256(f.read() + ARCH.encode()).hexdigest()[:24]
+        key = hashlib.sha256(f.read() + ARCH.encode() + headers_digest().encode()).hexdigest()[:24]
     cached = os.path.join(locations.pluginCacheRoot(), "lib%s.so" % key)
     if not os.path.exists(cached):
         tmp = cached + ".tmp%d" % os.getpid()

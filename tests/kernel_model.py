@@ -84,7 +84,16 @@ def build_view(g):
         rec = [outPairPos[k] for k in range(outR[v_], outR[v_ + 1]) if outPairNbr[k] == u_ and outPairPos[k] > p]
         if rec:
             tf[5, p] = T[min(rec)]
+    # root gaps (tm_graph_view.rgap): first-candidate time minus the root's
+    # time, clamped to INT_MAX; 0xffffffff for self-loops and no candidate;
+    # padded to a multiple of 256 entries plus 256 with 0xffffffff
+    epad = (E + 255) // 256 * 256 + 256
+    rg = np.full((6, epad), 0xFFFFFFFF, dtype=np.int64)
+    ok = (Eg[:, 0] != Eg[:, 1])[None, :] & (tf != NONE)
+    d = np.minimum(tf - T[None, :].astype(np.int64), 2**31 - 1)
+    rg[:, :E][ok] = d[ok]
     return dict(E=E, V=V, Eg=Eg, T=Arr(T, "T"), tfirst=[Arr(tf[k], "tfirst%d" % k) for k in range(6)],
+                rgap=[Arr(rg[k], "rgap%d" % k) for k in range(6)], epad=epad,
                 outSelf=Arr(outSelf, "outSelf"), inSelf=Arr(inSelf, "inSelf"),
                 outNext=Arr(outNext, "outNext"), inNext=Arr(inNext, "inNext"),
                 outPairSelf=Arr(outPairSelf, "outPairSelf"),
@@ -97,6 +106,77 @@ def build_view(g):
 
 
 S_SEARCH, S_OUT_SELF, S_OUT_NEXT, S_IN_SELF, S_IN_NEXT, S_PAIR_SELF = range(6)
+TM_BLOCK = 4 * WAVE
+
+
+def root_passes(g, md, delta, r):
+    """Root filter of the dispatch: the root's gap under edge 1's start rule
+    is within delta (unsigned compare; self-loops carry 0xffffffff)."""
+    return delta >= 0 and g["rgap"][md["tmode"][1]][r] <= delta
+
+
+def batch_dispatch(g, queries, chunk_roots=1024, nwaves=3, instr=False):
+    """Model of the multi-query root dispatch of one launch.
+
+    queries: list of (motif descriptor, delta, root_beg, root_end).  Mirrors
+    the host plan (dispatch order sorted by start rule then decreasing delta,
+    rule ranges, the union of root ranges aligned down to 4, chunking) and the
+    kernel's block / quarter / rule / entry walk, with wavefronts taking
+    chunks in turn.  Returns the list of (query, root) pairs a wavefront
+    expands with that lane passing (instr: every in-range lane)."""
+    nq = len(queries)
+    root_lo = min(q[2] for q in queries)
+    root_hi = max(max(q[3] for q in queries), root_lo)
+    for md, _, _, _ in queries:
+        assert 0 <= md["tmode"][1] < 6
+    order = sorted(range(nq), key=lambda i: (queries[i][0]["tmode"][1], -queries[i][1]))
+    rule_first, rule_end, rule_maxdelta, mask = [0] * 6, [0] * 6, [-1] * 6, 0
+    qtab = []
+    for i, qi in enumerate(order):
+        k = queries[qi][0]["tmode"][1]
+        if not (mask >> k) & 1:
+            mask |= 1 << k
+            rule_first[k] = i
+            rule_maxdelta[k] = queries[qi][1]
+        rule_end[k] = i + 1
+        qtab.append((qi, queries[qi][1], queries[qi][2], queries[qi][3]))
+    root_al = root_lo & ~3
+    assert chunk_roots % TM_BLOCK == 0
+    total_chunks = (root_hi - root_al + chunk_roots - 1) // chunk_roots
+    out = []
+    for c in range(total_chunks):
+        first = root_al + c * chunk_roots
+        nblk = min(chunk_roots // TM_BLOCK, (root_hi - first + TM_BLOCK - 1) // TM_BLOCK)
+        for b in range(nblk):
+            blk = first + b * TM_BLOCK
+            near = False
+            for k in range(6):
+                if not (mask >> k) & 1:
+                    continue
+                for lane in range(WAVE):
+                    r0 = blk + 4 * lane
+                    assert r0 % 4 == 0 and r0 + 3 < g["epad"]  # one aligned 16-byte load
+                    x = [g["rgap"][k][r0 + i] if root_lo <= r0 + i < root_hi else 0xFFFFFFFF for i in range(4)]
+                    near |= rule_maxdelta[k] >= 0 and min(x) <= rule_maxdelta[k]
+            if not near and not instr:
+                continue
+            for quarter in range(4):
+                for k in range(6):
+                    if not (mask >> k) & 1:
+                        continue
+                    rs = [blk + 4 * lane + quarter for lane in range(WAVE)]
+                    lg = [g["rgap"][k][r] if root_lo <= r < root_hi else 0xFFFFFFFF for r in rs]
+                    smin = min(lg)
+                    for e in range(rule_first[k], rule_end[k]):
+                        qid, dl, qb, qe = qtab[e]
+                        if not instr and (dl < 0 or smin > dl):
+                            break
+                        for lane, r in enumerate(rs):
+                            inq = root_lo <= r < root_hi and qb <= r < qe
+                            ps = inq and dl >= 0 and lg[lane] <= dl
+                            if ps or (instr and inq):
+                                out.append((qid, r))
+    return out
 
 
 def build_motif(edges):
@@ -311,7 +391,7 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                     continue
                 m = [0] * MAXN
                 m[0], m[1] = u, v
-                if g["tfirst"][md["tmode"][1]][r] > t + delta and not instr:
+                if not root_passes(g, md, delta, r) and not instr:
                     continue
                 L = cand_start(g, md, 1, m, r)
                 if instr:
@@ -425,7 +505,8 @@ class _Shared:
         self.next_chunk = [0] * STRIPES   # striped chunk counters
         self.nchunks = nchunks
         self.pool = []          # donated frames, in reservation order
-        self.head = 0
+        self.head = 0           # tickets drawn
+        self.taken = set()
         self.cap = pool_cap
         self.busy = 0
         self.idle = 0
@@ -449,6 +530,7 @@ class Wave:
         self.next_root = 0
         self.roots_done = False
         self.is_idle = False
+        self.ticket = None
         self.steps = 0
         self.done = False
         self.home = Wave.count % STRIPES
@@ -461,16 +543,27 @@ class Wave:
         M = md["M"]
         last = M - 2
         if self.top == 0 and self.roots_done:
+            # ticket protocol: draw a ticket, wait for its record or for
+            # busy == 0 (busy counts holders of work plus untaken records)
             if not self.is_idle:
                 sh.busy -= 1
                 sh.idle += 1
                 self.is_idle = True
-            if sh.head < len(sh.pool) and sh.head < sh.cap:
-                sh.busy += 1
-                f = sh.pool[sh.head]
+                self.ticket = None
+            if self.ticket is None:
+                self.ticket = sh.head
                 sh.head += 1
+                if self.ticket >= sh.cap:
+                    sh.idle -= 1
+                    self.done = True
+                return
+            if self.ticket < len(sh.pool):
+                assert self.ticket not in sh.taken
+                sh.taken.add(self.ticket)
+                f = sh.pool[self.ticket]
+                self.ticket = None
                 sh.idle -= 1
-                self.is_idle = False
+                self.is_idle = False  # inherits the record's busy credit
                 self.frames[0] = dict(f)
                 self.top = 1
                 return
@@ -506,7 +599,7 @@ class Wave:
                     continue
                 m = [0] * 8
                 m[0], m[1] = u, v
-                if g["tfirst"][md["tmode"][1]][r] > t + self.delta:
+                if not root_passes(g, md, self.delta, r):
                     continue
                 L = cand_start(g, md, 1, m, r)
                 if not (L[2] < L[3] and g["T"][L[0][L[2]]] <= t + self.delta):
@@ -595,14 +688,16 @@ class Wave:
         self.steps += 1
         # donation
         if self.top > 0 and self.steps % self.steal_every == 0:
-            want = min(sh.idle - (min(len(sh.pool), sh.cap) - sh.head), self.steal_max)
+            want = min(min(sh.head, sh.cap) - len(sh.pool), self.steal_max)
             for _ in range(max(0, want)):
                 cand = [i for i in range(min(self.top, WAVE)) if frames[i]["end"] - frames[i]["cur"] >= 2 * self.steal_min]
                 if not cand:
                     break
                 b = cand[0]
+                sh.busy += 1  # the record's credit, before it can be seen
                 t = len(sh.pool)
                 if t >= sh.cap:
+                    sh.busy -= 1
                     break
                 f = dict(frames[b])
                 mid = f["cur"] + (f["end"] - f["cur"]) // 2
@@ -629,5 +724,6 @@ def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, ste
         w.step()
         guard += 1
         assert guard < 10_000_000, "no progress"
-    assert sh.head == min(len(sh.pool), sh.cap)
+    assert len(sh.taken) == len(sh.pool), "a donated record was never taken"
+    assert sh.busy == 0 and sh.idle == 0
     return sum(w.cnt for w in waves), len(sh.pool)

@@ -82,3 +82,37 @@ def test_model_work_donation_between_wavefronts():
         assert got == want
         donated += nd
     assert donated > 0
+
+
+def test_model_batch_dispatch_covers_every_match_once():
+    """The multi-query root dispatch (blocks of 256 roots, gap filter, queries
+    sorted by start rule and decreasing delta) expands each (query, root) pair
+    at most once and never drops a root that has a match: per-query sums of
+    single-root oracle counts over the dispatched roots equal the full counts."""
+    rng = np.random.default_rng(31)
+    for it in range(12):
+        E = int(rng.integers(1, 700))
+        U, V, T = random_graph(rng, E, int(rng.integers(2, 14)), int(rng.integers(1, 300)))
+        g = prep.preprocess(U, V, T)
+        gv = km.build_view(g)
+        E = g["E"]
+        qs, mots = [], []
+        for _ in range(int(rng.integers(1, 9))):
+            M = int(rng.integers(2, 6))
+            mot = random_motif(rng, M)
+            d = int(rng.integers(-2, 60))
+            b = int(rng.integers(0, E + 1))
+            e = int(rng.integers(b, E + 1)) if rng.random() < 0.7 else E
+            qs.append((km.build_motif([(a, c) for a, c, _ in mot]), d, b, e))
+            mots.append(mot)
+        pairs = km.batch_dispatch(gv, qs, chunk_roots=int(rng.choice([256, 512, 1024])))
+        assert len(pairs) == len(set(pairs))
+        for qi, (md, d, b, e) in enumerate(qs):
+            roots = [r for (q, r) in pairs if q == qi]
+            assert all(b <= r < e for r in roots)
+            got = sum(oracle.count(g, mots[qi], d, r, r + 1, nthreads=1) for r in roots)
+            assert got == oracle.count(g, mots[qi], d, b, e, nthreads=1), (it, qi)
+        # instrumented launches expand every in-range root of every query
+        ipairs = km.batch_dispatch(gv, qs, chunk_roots=256, instr=True)
+        for qi, (md, d, b, e) in enumerate(qs):
+            assert sorted(r for (q, r) in ipairs if q == qi) == list(range(b, e))

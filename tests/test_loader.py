@@ -63,7 +63,7 @@ def test_edge_file_and_caches(tmp_path):
 
 def test_motif_reader_and_predecode():
     for f in sorted(os.listdir(os.path.join(ROOT, "inputs", "motifs"))):
-        p = os.path.join(ROOT, "tests", "golden", "ref_motifs", f)
+        p = os.path.join(ROOT, "inputs", "motifs", f)
         m = core.data.HostMotifData(p)
         edges = omotif.read_motif(p)
         assert m.edges() == edges
