@@ -44,7 +44,7 @@ def main():
     ap.add_argument("--graph", default="pl64m")
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--qlimit-ms", type=int, default=5000, help="per-query budget in the plain pass")
+    ap.add_argument("--qlimit-ms", type=int, default=2000, help="per-query budget in the plain pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget (0 = skip)")
     ap.add_argument("--json-out", default="")
     args = ap.parse_args()
@@ -92,6 +92,8 @@ def main():
         plain_ms.append((time.perf_counter() - t) * 1e3)
         kernel_ms.append(dg.last_kernel_ms())
         plain_counts.append((i, c))
+        if (i + 1) % 32 == 0:
+            log("plain: %d/%d queries, %.1fs so far" % (i + 1, len(motifs), sum(plain_ms) / 1e3))
     keep = [i for i, _ in plain_counts]
     plain_total_s = sum(plain_ms) / 1e3
     plain_qps = len(keep) / plain_total_s
@@ -99,10 +101,13 @@ def main():
 
     # ---- reference work of the kept queries (instrumented kernel, untimed)
     work = np.zeros(4, dtype=np.float64)
-    for i in keep:
+    t = time.perf_counter()
+    for n, i in enumerate(keep):
         c, w = dg.count_work(motifs[i][0], motifs[i][1])
         assert c == dict(plain_counts)[i]
         work += np.array(w, dtype=np.float64)
+        if (n + 1) % 32 == 0:
+            log("reference work: %d/%d queries, %.1fs" % (n + 1, len(keep), time.perf_counter() - t))
     b_alg = 12 * work[0] + 8 * work[1] + 16 * work[2] + 16 * work[3]
     log("reference work: R=%d X1=%d X2=%d P=%d -> %.3f GB" % (*work, b_alg / 1e9))
 

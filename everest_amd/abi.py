@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libtm_hip.so")
+# EVEREST_TM_LIB points at another build of the same library (tuning experiments)
+LIB_PATH = os.environ.get("EVEREST_TM_LIB") or os.path.join(LIB_DIR, "libtm_hip.so")
 MAX_E = 8
 
 TM_OK = 0
@@ -32,10 +33,11 @@ class HostGraph(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_roots", ctypes.c_int32),
+    _fields_ = [("waves_per_cu", ctypes.c_int32), ("claim_blocks", ctypes.c_int32),
                 ("scan_max", ctypes.c_int32), ("time_limit_ms", ctypes.c_int32),
                 ("steal_every", ctypes.c_int32), ("steal_max", ctypes.c_int32),
-                ("steal_min", ctypes.c_int32), ("steal_after_us", ctypes.c_int32)]
+                ("steal_min", ctypes.c_int32), ("steal_after_us", ctypes.c_int32),
+                ("waves_per_simd_hint", ctypes.c_int32)]
 
 
 _lib = None
@@ -78,13 +80,21 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def tuning(waves_per_cu=0, chunk_roots=0, scan_max=0, time_limit_ms=0, steal_every=0, steal_max=0, steal_min=0,
-           steal_after_us=0):
+TUNING_FIELDS = [f for f, _ in Tuning._fields_]
+
+
+def tuning(**kw):
+    """A Tuning with the given fields set (zero fields take library defaults)."""
     t = Tuning()
-    t.waves_per_cu, t.chunk_roots, t.scan_max = waves_per_cu, chunk_roots, scan_max
-    t.time_limit_ms, t.steal_every, t.steal_max, t.steal_min = time_limit_ms, steal_every, steal_max, steal_min
-    t.steal_after_us = steal_after_us
+    for k, v in kw.items():
+        if k not in TUNING_FIELDS:
+            raise TypeError("unknown tuning field %r" % k)
+        setattr(t, k, int(v))
     return t
+
+
+def tuning_dict(t):
+    return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
 
 
 def motif_arrays(edges):
@@ -123,7 +133,8 @@ class DeviceGraph:
         st = np.zeros(16, dtype=np.uint64)
         check(lib().tm_last_stats(self.handle, _p(st), 16))
         return dict(steps=int(st[0]), items=int(st[1]), roots=int(st[2]), leaves=int(st[3]), pushes=int(st[4]),
-                    donated=int(st[8]), taken=int(st[9]), capexit=int(st[10]), polls=int(st[11]))
+                    donated=int(st[8]), taken=int(st[9]), capexit=int(st[10]), polls=int(st[11]),
+                    grid=int(st[12]), claim_blocks=int(st[13]), lds=int(st[14]), wpe=int(st[15]))
 
     def last_kernel_ms(self):
         ms = ctypes.c_double()

@@ -153,7 +153,7 @@ __global__ void bucket_kernel(const int *__restrict__ T, int E, int tmin, int bs
   bucket[k] = lo;
 }
 
-__global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops, int n) {
+__global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops, int n, int grid) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < TM_STRIPES) s->next_chunk[i][0] = 0;
   if (i == 0) {
@@ -161,12 +161,32 @@ __global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops
     s->abort = 0;
     s->pool_head = 0;
     s->pool_tail = 0;
-    s->busy = 0;
-    s->idle = 0;
+    s->busy = grid;  // every wavefront starts out holding (root) work
     s->t0 = (long long)wall_clock64();
   }
   if (i < 16) s->stats[i] = 0;
   if (i < n) { counts[i] = 0; stops[i] = 0; }
+}
+
+__global__ void clear_pool_kernel(unsigned long long *pool, size_t nw, unsigned *done, size_t nd) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nw) pool[i] = 0;
+  if (i < nd) done[i] = 0;
+}
+
+// Sum the per-wavefront counters of a launch into sched->stats (one block).
+__global__ void __launch_bounds__(256) stats_kernel(const unsigned long long *__restrict__ w, int grid, tm_sched *s) {
+  __shared__ unsigned long long acc[256];
+  const int k = threadIdx.x % TM_NSTATS, part = threadIdx.x / TM_NSTATS;  // 16 partial sums per counter
+  unsigned long long v = 0;
+  for (int b = part; b < grid; b += 256 / TM_NSTATS) v += w[(size_t)b * TM_NSTATS + k];
+  acc[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x < TM_NSTATS) {
+    unsigned long long t = 0;
+    for (int p = 0; p < 256 / TM_NSTATS; p++) t += acc[p * TM_NSTATS + threadIdx.x];
+    s->stats[threadIdx.x] = t;
+  }
 }
 
 int ceil_log2(long long x) {
@@ -184,16 +204,12 @@ struct tm_graph {
   int E = 0, V = 0;
   int num_cus = 0;
   long long wall_rate_khz = 100000;  // wall_clock64 ticks per millisecond
-  hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   int64_t bytes = 0;
   tm_graph_view view{};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  float last_kernel_ms = 0.f;
-  int last_chunk_roots = 0, last_grid = 0;
-  tmh::Scratch scratch;
-  unsigned long long last_stats[16] = {};
-  std::mutex mtx;  // one launch at a time per graph handle
+  tmh::Slot slot[TM_SLOTS];  // slot 0 also carries the upload
+  int last_slot = 0;         // slot of the last finished launch (stats, kernel time)
+  std::mutex mtx;            // one API call at a time per graph handle
 };
 
 namespace tmh {
@@ -251,77 +267,64 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
   return TM_OK;
 }
 
-template <int MAXN, bool ENUM, bool INSTR>
+template <int MAXN, bool ENUM, bool INSTR, int WPE>
 static tm_status occupancy(int lds, int *blocks) {
-  HIPTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR>, 64,
-                                                      lds));
+  HIPTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks, tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE>, 64, lds));
   return TM_OK;
 }
 
-template <int MAXN, bool ENUM, bool INSTR>
-static tm_status launch_t(tm_graph *g, const tm_kernel_args &a, int grid, int lds) {
-  HIPTRY(hipEventRecord(g->ev0, g->stream));
-  hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR>), dim3(grid), dim3(64), lds, g->stream, a);
+template <int MAXN, bool ENUM, bool INSTR, int WPE>
+static tm_status launch_t(Slot &sl, const tm_kernel_args &a, int grid, int lds) {
+  HIPTRY(hipEventRecord(sl.ev0, sl.stream));
+  hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE>), dim3(grid), dim3(64), lds,
+                     sl.stream, a);
   HIPTRY(hipGetLastError());
-  HIPTRY(hipEventRecord(g->ev1, g->stream));
+  HIPTRY(hipEventRecord(sl.ev1, sl.stream));
   return TM_OK;
 }
 
-// one of the five kernel variants: (nodes <= 4 | <= 8) x (count | enumerate),
-// plus the instrumented count
+// Kernel variants: node map of 4 or 8, count / enumerate / instrumented
+// count; counting also comes with a register cap of 80 VGPRs (wpe 6) for
+// launches whose LDS stack leaves room for more than 20 wavefronts per CU.
+// OCC: report the occupancy in *blocks instead of launching.
+template <bool OCC, int MAXN, bool ENUM, bool INSTR, int WPE>
+static tm_status variant(Slot &sl, int lds, int *blocks, const tm_kernel_args *a, int grid) {
+  if (OCC) return occupancy<MAXN, ENUM, INSTR, WPE>(lds, blocks);
+  return launch_t<MAXN, ENUM, INSTR, WPE>(sl, *a, grid, lds);
+}
+
+template <bool OCC, int MAXN>
+static tm_status dispatch_n(Slot &sl, bool enumerate, bool instr, int wpe, int lds, int *blocks,
+                            const tm_kernel_args *a, int grid) {
+  if (instr) return variant<OCC, MAXN, false, true, 1>(sl, lds, blocks, a, grid);
+  if (enumerate) return variant<OCC, MAXN, true, false, 1>(sl, lds, blocks, a, grid);
+  if (wpe == 6) return variant<OCC, MAXN, false, false, 6>(sl, lds, blocks, a, grid);
+  return variant<OCC, MAXN, false, false, 1>(sl, lds, blocks, a, grid);
+}
+
 template <bool OCC>
-static tm_status dispatch(tm_graph *g, int nmap, bool enumerate, bool instr, int lds, int *blocks,
+static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wpe, int lds, int *blocks,
                           const tm_kernel_args *a, int grid) {
-  if (OCC) {
-    if (instr) return nmap == 4 ? occupancy<4, false, true>(lds, blocks) : occupancy<8, false, true>(lds, blocks);
-    if (nmap == 4) return enumerate ? occupancy<4, true, false>(lds, blocks) : occupancy<4, false, false>(lds, blocks);
-    return enumerate ? occupancy<8, true, false>(lds, blocks) : occupancy<8, false, false>(lds, blocks);
-  }
-  if (instr) return nmap == 4 ? launch_t<4, false, true>(g, *a, grid, lds) : launch_t<8, false, true>(g, *a, grid, lds);
-  if (nmap == 4)
-    return enumerate ? launch_t<4, true, false>(g, *a, grid, lds) : launch_t<4, false, false>(g, *a, grid, lds);
-  return enumerate ? launch_t<8, true, false>(g, *a, grid, lds) : launch_t<8, false, false>(g, *a, grid, lds);
+  return nmap == 4 ? dispatch_n<OCC, 4>(sl, enumerate, instr, wpe, lds, blocks, a, grid)
+                   : dispatch_n<OCC, 8>(sl, enumerate, instr, wpe, lds, blocks, a, grid);
 }
 
 // Plan and launch the mining kernel for staged queries [q0, q0 + nq) of
 // scratch.h_queries (nq <= TM_LAUNCH_MAXQ): size the persistent grid from
 // occupancy, pick the chunk size over the union of the root ranges, upload,
 // reset, launch.
-tm_status launch(tm_graph *g, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
+tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
                  tm_launch_fn fn = nullptr, void *user = nullptr) {
-  Scratch &s = g->scratch;
-  int maxM = 2, maxN = 2, root_lo = INT32_MAX, root_hi = 0;
-  std::vector<int> order(nq);
+  Scratch &s = sl.s;
+  int maxM = 2, maxN = 2;
+  long long span = 0;  // roots over all queries (aligned down to 4 per query)
   for (int i = 0; i < nq; i++) {
     const tm_query_dev &qd = s.h_queries[q0 + i];
     if (qd.m.tmode[1] < 0 || qd.m.tmode[1] >= TM_NTFIRST) return fail(TM_ERR_INTERNAL, "edge 1 has no start rule");
     maxM = std::max(maxM, qd.m.M);
     maxN = std::max(maxN, qd.m.n);
-    root_lo = std::min(root_lo, qd.root_beg);
-    root_hi = std::max(root_hi, qd.root_end);
-    order[i] = i;
-  }
-  if (root_hi < root_lo) root_hi = root_lo;
-  // dispatch order: by start rule of edge 1, then by decreasing delta, so a
-  // wavefront stops scanning a rule's queries at the first delta below the
-  // smallest gap of its roots
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    const tm_query_dev &a = s.h_queries[q0 + x], &b = s.h_queries[q0 + y];
-    if (a.m.tmode[1] != b.m.tmode[1]) return a.m.tmode[1] < b.m.tmode[1];
-    return a.delta > b.delta;
-  });
-  int rule_mask = 0, rule_first[TM_NTFIRST], rule_end[TM_NTFIRST], rule_maxdelta[TM_NTFIRST];
-  for (int k = 0; k < TM_NTFIRST; k++) { rule_first[k] = rule_end[k] = 0; rule_maxdelta[k] = -1; }
-  for (int i = 0; i < nq; i++) {
-    const tm_query_dev &qd = s.h_queries[q0 + order[i]];
-    const int k = qd.m.tmode[1];
-    if (!((rule_mask >> k) & 1)) { rule_mask |= 1 << k; rule_first[k] = i; rule_maxdelta[k] = qd.delta; }
-    rule_end[k] = i + 1;
-    int *qt = &s.h_qtab[4 * (size_t)(q0 + i)];
-    qt[0] = order[i];
-    qt[1] = qd.delta;
-    qt[2] = qd.root_beg;
-    qt[3] = qd.root_end;
+    if (qd.root_end > qd.root_beg) span += qd.root_end - (qd.root_beg & ~3);
   }
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
@@ -331,88 +334,124 @@ tm_status launch(tm_graph *g, int q0, int nq, bool enumerate, const tm_tuning *t
   const int lds = frames * per_frame * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
   int blocks = 0;
   tm_status st;
+  int wpe = (tune && tune->waves_per_simd_hint > 0) ? (tune->waves_per_simd_hint >= 6 ? 6 : 1) : 0;
   if (fn) {
     int32_t b = 0;
-    if (fn(0, nullptr, 0, lds, g->stream, &b, user) != 0) return fail(TM_ERR_HIP, "custom kernel occupancy query failed");
+    if (fn(0, nullptr, 0, lds, sl.stream, &b, user) != 0) return fail(TM_ERR_HIP, "custom kernel occupancy query failed");
     blocks = b;
-  } else if ((st = dispatch<true>(g, nmap, enumerate, instr, lds, &blocks, nullptr, 0)) != TM_OK) {
-    return st;
+  } else {
+    if (wpe == 0) {  // auto: the register cap pays off only where LDS is not the limit
+      int b6 = 0;
+      if ((st = dispatch<true>(sl, nmap, enumerate, instr, 6, lds, &b6, nullptr, 0)) != TM_OK) return st;
+      wpe = b6 >= 24 ? 6 : 1;
+    }
+    if ((st = dispatch<true>(sl, nmap, enumerate, instr, wpe, lds, &blocks, nullptr, 0)) != TM_OK) return st;
   }
   int wpc = (tune && tune->waves_per_cu > 0) ? tune->waves_per_cu : 32;
   if (blocks > 0) wpc = std::min(wpc, blocks);
   if (wpc < 1) wpc = 1;
-  int grid = g->num_cus * wpc;
+  int grid = std::min(g->num_cus * wpc, s.wstats_cap);
 
-  const int root_al = root_lo & ~3;
-  const long long roots = (long long)root_hi - root_al;
-  int chunk_roots;
-  if (tune && tune->chunk_roots > 0) {
-    chunk_roots = (tune->chunk_roots + 255) / 256 * 256;
-  } else {
-    // about 8 chunks per wavefront: few atomics, still fine-grained at the end
-    long long c = (roots + (long long)grid * 8 - 1) / ((long long)grid * 8);
-    c = (c + 255) / 256 * 256;
-    chunk_roots = (int)std::min<long long>(std::max<long long>(c, 1024), 16384);
+  long long total_blocks = 0, max_blocks = 0;
+  for (int i = 0; i < nq; i++) {
+    const tm_query_dev &qd = s.h_queries[q0 + i];
+    int *qt = &s.h_qtab[4 * (size_t)(q0 + i)];
+    qt[0] = qd.delta;
+    qt[1] = qd.root_beg;
+    qt[2] = qd.root_end;
+    qt[3] = qd.m.tmode[1];
+    s.h_prefix[q0 + i] = total_blocks;
+    if (qd.root_end > qd.root_beg) {
+      const long long nb = (qd.root_end - (qd.root_beg & ~3) + 255) / 256;
+      total_blocks += nb;
+      max_blocks = std::max(max_blocks, nb);
+    }
   }
-  const long long total_chunks = (roots + chunk_roots - 1) / chunk_roots;
-  if (total_chunks < grid) grid = (int)std::max<long long>(1, total_chunks);
+  s.h_prefix[q0 + nq] = total_blocks;
+  if (total_blocks < grid) grid = (int)std::max<long long>(1, total_blocks);
+  // claims of about 1/8 of one query's roots per wavefront: as fine-grained
+  // as a launch of that query alone, whatever the number of queries
+  int claim_blocks;
+  if (tune && tune->claim_blocks > 0) {
+    claim_blocks = tune->claim_blocks;
+  } else {
+    const long long c = (max_blocks + (long long)grid * 8 - 1) / ((long long)grid * 8);
+    claim_blocks = (int)std::min<long long>(std::max<long long>(c, 4), 64);
+  }
 
   HIPTRY(hipMemcpyAsync(s.d_queries + q0, s.h_queries.data() + q0, sizeof(tm_query_dev) * nq,
-                        hipMemcpyHostToDevice, g->stream));
+                        hipMemcpyHostToDevice, sl.stream));
   HIPTRY(hipMemcpyAsync(s.d_qtab + 4 * (size_t)q0, s.h_qtab.data() + 4 * (size_t)q0, sizeof(int) * 4 * nq,
-                        hipMemcpyHostToDevice, g->stream));
+                        hipMemcpyHostToDevice, sl.stream));
+  HIPTRY(hipMemcpyAsync(s.d_prefix + q0, s.h_prefix.data() + q0, sizeof(long long) * (nq + 1),
+                        hipMemcpyHostToDevice, sl.stream));
 
   tm_kernel_args a;
   memset(&a, 0, sizeof(a));
   a.g = g->view;
   a.queries = s.d_queries + q0;
   a.qtab = s.d_qtab + 4 * (size_t)q0;
+  a.block_prefix = s.d_prefix + q0;
   a.nq = nq;
-  a.chunk_roots = chunk_roots;
-  a.root_lo = root_lo;
-  a.root_hi = root_hi;
-  a.root_al = root_al;
-  a.rule_mask = rule_mask;
-  for (int k = 0; k < TM_NTFIRST; k++) {
-    a.rule_first[k] = rule_first[k];
-    a.rule_end[k] = rule_end[k];
-    a.rule_maxdelta[k] = rule_maxdelta[k];
-  }
-  a.total_chunks = total_chunks;
+  a.total_blocks = total_blocks;
+  a.claim_blocks = claim_blocks;
   a.frames = frames;
   a.scan_max = (tune && tune->scan_max > 0) ? tune->scan_max : 16;
   a.limit_ticks = 0;
   if (tune && tune->time_limit_ms > 0) a.limit_ticks = (long long)tune->time_limit_ms * (g->wall_rate_khz);
   a.sched = s.d_sched;
+  a.wstats = s.d_wstats;
+  if (++s.gen >= (1 << TM_GEN_BITS)) {
+    // generations wrap: clear the ring so no stale tag can match (a kernel,
+    // not a memset, so this is also safe inside a captured stream)
+    const size_t nw = (size_t)TM_POOL_FIELDS * s.pool_cap;
+    hipLaunchKernelGGL(clear_pool_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, sl.stream, s.d_pool, nw,
+                       s.d_pool_done, (size_t)s.pool_cap);
+    HIPTRY(hipGetLastError());
+    s.gen = 1;
+  }
   a.pool = s.d_pool;
+  a.pool_done = s.d_pool_done;
   a.pool_cap = s.pool_cap;
-  a.gen = ++s.gen;
+  a.pool_shift = s.pool_shift;
+  a.gen = s.gen;
   a.steal_every = (tune && tune->steal_every > 0) ? tune->steal_every : 16;
   a.steal_max = (tune && tune->steal_max > 0) ? tune->steal_max : 4;
   a.idle_cap = std::max(64, g->num_cus / 2);
   a.steal_min = (tune && tune->steal_min > 0) ? tune->steal_min : 64;
   {
-    const long long us = (tune && tune->steal_after_us > 0) ? tune->steal_after_us : 200;
+    const long long us = (tune && tune->steal_after_us > 0) ? tune->steal_after_us : 1000;
     a.steal_after_ticks = us * g->wall_rate_khz / 1000;
   }
-  g->last_chunk_roots = chunk_roots;
-  g->last_grid = grid;
+  sl.last_claim = claim_blocks;
+  sl.last_grid = grid;
+  sl.last_lds = lds;
+  sl.last_wpe = fn ? 0 : wpe;
 
-  hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, g->stream, s.d_sched,
-                     s.d_counts + q0, s.d_stops + q0, nq);
+  hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, sl.stream, s.d_sched,
+                     s.d_counts + q0, s.d_stops + q0, nq, grid);
   HIPTRY(hipGetLastError());
   if (fn) {
-    HIPTRY(hipEventRecord(g->ev0, g->stream));
-    if (fn(1, &a, grid, lds, g->stream, nullptr, user) != 0) return fail(TM_ERR_HIP, "custom kernel launch failed");
-    HIPTRY(hipEventRecord(g->ev1, g->stream));
-    return TM_OK;
+    HIPTRY(hipEventRecord(sl.ev0, sl.stream));
+    if (fn(1, &a, grid, lds, sl.stream, nullptr, user) != 0) return fail(TM_ERR_HIP, "custom kernel launch failed");
+    HIPTRY(hipEventRecord(sl.ev1, sl.stream));
+  } else if ((st = dispatch<false>(sl, nmap, enumerate, instr, wpe, lds, &blocks, &a, grid)) != TM_OK) {
+    return st;
   }
-  return dispatch<false>(g, nmap, enumerate, instr, lds, &blocks, &a, grid);
+  hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, sl.stream, s.d_wstats, grid, s.d_sched);
+  HIPTRY(hipGetLastError());
+  return TM_OK;
 }
 
-// Reserve query staging space for nq queries (grows, never shrinks).
-tm_status reserve(tm_graph *g, int nq) {
-  Scratch &s = g->scratch;
+// Create the slot's stream and reserve its staging space for nq queries
+// (grows, never shrinks).
+tm_status reserve(tm_graph *g, Slot &sl, int nq) {
+  Scratch &s = sl.s;
+  if (!sl.stream) {
+    HIPTRY(hipStreamCreate(&sl.stream));
+    HIPTRY(hipEventCreate(&sl.ev0));
+    HIPTRY(hipEventCreate(&sl.ev1));
+  }
   if (nq <= s.cap_q) return TM_OK;
   int cap = std::max(nq, 256);
   void *p;
@@ -421,6 +460,8 @@ tm_status reserve(tm_graph *g, int nq) {
   s.d_queries = (tm_query_dev *)p;
   if ((st = dev_alloc(g, &p, sizeof(int) * 4 * cap)) != TM_OK) return st;
   s.d_qtab = (int *)p;
+  if ((st = dev_alloc(g, &p, sizeof(long long) * (cap + 1))) != TM_OK) return st;
+  s.d_prefix = (long long *)p;
   if ((st = dev_alloc(g, &p, sizeof(unsigned long long) * cap)) != TM_OK) return st;
   s.d_counts = (unsigned long long *)p;
   if ((st = dev_alloc(g, &p, sizeof(int) * cap)) != TM_OK) return st;
@@ -428,16 +469,24 @@ tm_status reserve(tm_graph *g, int nq) {
   if (!s.d_sched) {
     if ((st = dev_alloc(g, &p, sizeof(tm_sched))) != TM_OK) return st;
     s.d_sched = (tm_sched *)p;
-    s.pool_cap = 1 << 18;
+    s.pool_shift = 18;
+    s.pool_cap = 1 << s.pool_shift;
     const size_t pool_bytes = sizeof(unsigned long long) * TM_POOL_FIELDS * (size_t)s.pool_cap;
     if ((st = dev_alloc(g, &p, pool_bytes)) != TM_OK) return st;
     s.d_pool = (unsigned long long *)p;
-    HIPTRY(hipMemsetAsync(s.d_pool, 0, pool_bytes, g->stream));  // generation 0 is never used
-    HIPTRY(hipStreamSynchronize(g->stream));
+    if ((st = dev_alloc(g, &p, sizeof(unsigned) * (size_t)s.pool_cap)) != TM_OK) return st;
+    s.d_pool_done = (unsigned *)p;
+    HIPTRY(hipMemsetAsync(s.d_pool_done, 0, sizeof(unsigned) * (size_t)s.pool_cap, sl.stream));
+    s.wstats_cap = g->num_cus * 32;  // the most resident one-wave workgroups
+    if ((st = dev_alloc(g, &p, sizeof(unsigned long long) * TM_NSTATS * (size_t)s.wstats_cap)) != TM_OK) return st;
+    s.d_wstats = (unsigned long long *)p;
+    HIPTRY(hipMemsetAsync(s.d_pool, 0, pool_bytes, sl.stream));  // generation 0 is never used
+    HIPTRY(hipStreamSynchronize(sl.stream));
   }
   s.cap_q = cap;
   s.h_queries.resize(cap);
   s.h_qtab.resize(4 * (size_t)cap);
+  s.h_prefix.resize(cap + 1);
   return TM_OK;
 }
 
@@ -498,16 +547,15 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   hipError_t he = hipGetDeviceProperties(&prop, device);
   if (he != hipSuccess) { delete g; return fail(TM_ERR_HIP, hipGetErrorString(he)); }
   g->num_cus = prop.multiProcessorCount;
+  tm_status st0;
   {
     int rate = 0;
     if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, device) == hipSuccess && rate > 0)
       g->wall_rate_khz = rate;
   }
-  he = hipStreamCreate(&g->stream);
-  if (he != hipSuccess) { delete g; return fail(TM_ERR_HIP, hipGetErrorString(he)); }
-  if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+  if ((st0 = reserve(g, g->slot[0], 1)) != TM_OK) {
     tm_graph_free(g);
-    return fail(TM_ERR_HIP, "hipEventCreate failed");
+    return st0;
   }
 
   auto cleanup = [&](tm_status s) {
@@ -558,7 +606,7 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   vw.nbuckets = nb;
   ALLOC(vw.bucket, sizeof(int) * (size_t)(nb + 1));
 #undef ALLOC
-  hipStream_t s = g->stream;
+  hipStream_t s = g->slot[0].stream;
 #define H2D(dst, src, bytes) \
   if ((he = hipMemcpyAsync((void *)(dst), (src), (bytes), hipMemcpyHostToDevice, s)) != hipSuccess) \
     return cleanup(fail(TM_ERR_HIP, hipGetErrorString(he)));
@@ -626,7 +674,6 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   TMPCHK(hipStreamSynchronize(s));
 #undef TMPCHK
   free_tmp();
-  if ((st = reserve(g, 1)) != TM_OK) return cleanup(st);
   *out = g;
   return TM_OK;
 }
@@ -634,24 +681,27 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
 tm_status tm_graph_free(tm_graph *g) {
   if (!g) return TM_OK;
   (void)hipSetDevice(g->device);
-  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (Slot &sl : g->slot)
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
   for (void *p : g->allocs) (void)hipFree(p);
-  if (g->ev0) (void)hipEventDestroy(g->ev0);
-  if (g->ev1) (void)hipEventDestroy(g->ev1);
-  if (g->stream) (void)hipStreamDestroy(g->stream);
+  for (Slot &sl : g->slot) {
+    if (sl.ev0) (void)hipEventDestroy(sl.ev0);
+    if (sl.ev1) (void)hipEventDestroy(sl.ev1);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
   delete g;
   return TM_OK;
 }
 
 tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n) {
   if (!g || !stats || n < 0) return fail(TM_ERR_ARG, "bad argument");
-  for (int i = 0; i < n && i < 16; i++) stats[i] = g->last_stats[i];
+  for (int i = 0; i < n && i < 16; i++) stats[i] = g->slot[g->last_slot].last_stats[i];
   return TM_OK;
 }
 
 tm_status tm_last_kernel_ms(const tm_graph *g, double *ms) {
   if (!g || !ms) return fail(TM_ERR_ARG, "bad argument");
-  *ms = g->last_kernel_ms;
+  *ms = g->slot[g->last_slot].last_kernel_ms;
   return TM_OK;
 }
 
@@ -665,14 +715,17 @@ tm_status tm_graph_info(const tm_graph *g, int64_t *device_bytes, int32_t *num_e
   return TM_OK;
 }
 
-static tm_status stage_queries(tm_graph *g, int nq, const int32_t *num_edges, const int32_t *msrc,
-                               const int32_t *mdst, int stride, const int32_t *deltas,
+// Stage queries idx[0..nq) (indexes into the caller's arrays) at staging
+// positions [q0, q0 + nq) of the slot.
+static tm_status stage_queries(tm_graph *g, Slot &sl, int q0, int nq, const int *idx, const int32_t *num_edges,
+                               const int32_t *msrc, const int32_t *mdst, int stride, const int32_t *deltas,
                                const int32_t *begs, const int32_t *ends) {
   tm_status st;
-  if ((st = reserve(g, nq)) != TM_OK) return st;
-  Scratch &s = g->scratch;
-  for (int q = 0; q < nq; q++) {
-    tm_query_dev &qd = s.h_queries[q];
+  if ((st = reserve(g, sl, q0 + nq)) != TM_OK) return st;
+  Scratch &s = sl.s;
+  for (int i = 0; i < nq; i++) {
+    const int q = idx ? idx[i] : i;
+    tm_query_dev &qd = s.h_queries[q0 + i];
     memset(&qd, 0, sizeof(qd));
     if ((st = build_motif(num_edges[q], msrc + (size_t)q * stride, mdst + (size_t)q * stride, &qd.m)) != TM_OK)
       return st;
@@ -686,21 +739,28 @@ static tm_status stage_queries(tm_graph *g, int nq, const int32_t *num_edges, co
     if (end < beg) end = beg;
     qd.root_beg = beg;
     qd.root_end = end;
-    qd.count = s.d_counts + q;
-    qd.stop = s.d_stops + q;
+    qd.count = s.d_counts + q0 + i;
+    qd.stop = s.d_stops + q0 + i;
   }
   return TM_OK;
 }
 
-static tm_status finish(tm_graph *g, int q0, int nq, uint64_t *counts) {
-  Scratch &s = g->scratch;
+// Wait for the slot's launch of staged queries [q0, q0 + nq); copy their
+// counts out and check the launch's error bits.
+static tm_status finish(tm_graph *g, Slot &sl, int q0, int nq, uint64_t *counts) {
+  Scratch &s = sl.s;
   tm_sched hs;
-  HIPTRY(hipMemcpyAsync(&hs, s.d_sched, sizeof(hs), hipMemcpyDeviceToHost, g->stream));
-  HIPTRY(hipMemcpyAsync(counts, s.d_counts + q0, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, g->stream));
-  HIPTRY(hipStreamSynchronize(g->stream));
-  memcpy(g->last_stats, hs.stats, sizeof(g->last_stats));
+  HIPTRY(hipMemcpyAsync(&hs, s.d_sched, sizeof(hs), hipMemcpyDeviceToHost, sl.stream));
+  HIPTRY(hipMemcpyAsync(counts, s.d_counts + q0, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, sl.stream));
+  HIPTRY(hipStreamSynchronize(sl.stream));
+  memcpy(sl.last_stats, hs.stats, sizeof(sl.last_stats));
+  sl.last_stats[TM_STAT_GRID] = (unsigned long long)sl.last_grid;
+  sl.last_stats[TM_STAT_CLAIM] = (unsigned long long)sl.last_claim;
+  sl.last_stats[TM_STAT_LDS] = (unsigned long long)sl.last_lds;
+  sl.last_stats[TM_STAT_WPE] = (unsigned long long)sl.last_wpe;
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) g->last_kernel_ms = ms;
+  if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) sl.last_kernel_ms = ms;
+  g->last_slot = (int)(&sl - g->slot);
   if (hs.error & 4) return fail(TM_ERR_TIMEOUT, "query exceeded its time limit; counts discarded");
   if (hs.error) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(hs.error) + ")");
   return TM_OK;
@@ -712,10 +772,11 @@ static tm_status count_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc,
   if (!g || !msrc || !mdst || !count) return fail(TM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
+  Slot &sl = g->slot[0];
+  tm_status st = stage_queries(g, sl, 0, 1, nullptr, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
-  if ((st = launch(g, 0, 1, false, tune, false, fn, user)) != TM_OK) return st;
-  return finish(g, 0, 1, count);
+  if ((st = launch(g, sl, 0, 1, false, tune, false, fn, user)) != TM_OK) return st;
+  return finish(g, sl, 0, 1, count);
 }
 
 tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
@@ -735,30 +796,71 @@ tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, con
   if (!g || !msrc || !mdst || !count || !work) return fail(TM_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
+  Slot &sl = g->slot[0];
+  tm_status st = stage_queries(g, sl, 0, 1, nullptr, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
-  if ((st = launch(g, 0, 1, false, tune, true)) != TM_OK) return st;
-  if ((st = finish(g, 0, 1, count)) != TM_OK) return st;
-  const tm_query_dev &q = g->scratch.h_queries[0];
+  if ((st = launch(g, sl, 0, 1, false, tune, true)) != TM_OK) return st;
+  if ((st = finish(g, sl, 0, 1, count)) != TM_OK) return st;
+  const tm_query_dev &q = sl.s.h_queries[0];
   work[0] = (uint64_t)(q.root_end - q.root_beg);
-  work[1] = g->last_stats[TM_STAT_REF_X1];
-  work[2] = g->last_stats[TM_STAT_REF_X2];
-  work[3] = g->last_stats[TM_STAT_REF_P];
+  work[1] = sl.last_stats[TM_STAT_REF_X1];
+  work[2] = sl.last_stats[TM_STAT_REF_X2];
+  work[3] = sl.last_stats[TM_STAT_REF_P];
   return TM_OK;
 }
+
+// Launch shape class of a count query: frame-stack levels and node-map width
+// (they set the LDS a wavefront needs, hence the occupancy of its launch).
+static int shape_class(int M, int n) { return std::max(1, M - 2) * 2 + (n <= 4 ? 0 : 1); }
 
 tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, const int32_t *msrc,
                          const int32_t *mdst, const int32_t *deltas, const tm_tuning *tune, uint64_t *counts) {
   if (!g || nq < 1 || !num_edges || !msrc || !mdst || !deltas || !counts) return fail(TM_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  tm_status st = stage_queries(g, nq, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas, nullptr, nullptr);
-  if (st != TM_OK) return st;
-  for (int q0 = 0; q0 < nq; q0 += TM_LAUNCH_MAXQ) {
-    const int n = std::min(TM_LAUNCH_MAXQ, nq - q0);
-    if ((st = launch(g, q0, n, false, tune, false)) != TM_OK) return st;
-    if ((st = finish(g, q0, n, counts + q0)) != TM_OK) return st;
+  // Queries are grouped by shape class, so no launch sizes its LDS stack for
+  // a deeper motif than its own; groups (at most TM_LAUNCH_MAXQ queries) run
+  // concurrently on the slots' streams, TM_SLOTS at a time, and a wavefront
+  // that runs out of work in one launch frees its CU for the others.
+  std::vector<int> key(nq), order(nq);
+  for (int q = 0; q < nq; q++) {
+    if (num_edges[q] < 2 || num_edges[q] > TM_MAX_MOTIF_EDGES) return fail(TM_ERR_MOTIF, "motif needs 2..8 edges");
+    int n = 0;
+    for (int k = 0; k < num_edges[q]; k++)
+      n = std::max(n, 1 + std::max(msrc[(size_t)q * TM_MAX_MOTIF_EDGES + k], mdst[(size_t)q * TM_MAX_MOTIF_EDGES + k]));
+    key[q] = shape_class(num_edges[q], n);
+    order[q] = q;
   }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  struct Group { int first, n; };
+  std::vector<Group> groups;
+  for (int i = 0; i < nq;) {
+    int j = i;
+    while (j < nq && key[order[j]] == key[order[i]] && j - i < TM_LAUNCH_MAXQ) j++;
+    groups.push_back({i, j - i});
+    i = j;
+  }
+  std::vector<uint64_t> out(nq);
+  tm_status st = TM_OK;
+  for (size_t r = 0; r < groups.size(); r += TM_SLOTS) {
+    const size_t rn = std::min<size_t>(TM_SLOTS, groups.size() - r);
+    size_t launched = 0;
+    for (size_t k = 0; k < rn && st == TM_OK; k++) {
+      const Group &gr = groups[r + k];
+      Slot &sl = g->slot[k];
+      st = stage_queries(g, sl, 0, gr.n, order.data() + gr.first, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas,
+                         nullptr, nullptr);
+      if (st == TM_OK) st = launch(g, sl, 0, gr.n, false, tune, false);
+      if (st == TM_OK) launched++;
+    }
+    for (size_t k = 0; k < launched; k++) {  // wait for every launched group, even after an error
+      const Group &gr = groups[r + k];
+      const tm_status fs = finish(g, g->slot[k], 0, gr.n, out.data() + gr.first);
+      if (st == TM_OK) st = fs;
+    }
+    if (st != TM_OK) return st;
+  }
+  for (int i = 0; i < nq; i++) counts[order[i]] = out[i];
   return TM_OK;
 }
 
@@ -785,16 +887,17 @@ static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, 
   if (!g || !msrc || !mdst || !count || cap < 0 || (cap > 0 && !rows)) return fail(TM_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
-  tm_status st = stage_queries(g, 1, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
+  Slot &sl = g->slot[0];
+  tm_status st = stage_queries(g, sl, 0, 1, nullptr, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
-  Scratch &s = g->scratch;
+  Scratch &s = sl.s;
   int32_t *d_rows = nullptr;
   const size_t row_bytes = sizeof(int32_t) * (size_t)std::max<int64_t>(cap, 1) * num_edges;
   HIPTRY(hipMalloc(&d_rows, row_bytes));
   s.h_queries[0].rows = d_rows;
   s.h_queries[0].cap = cap;
-  if ((st = launch(g, 0, 1, true, tune, false, fn, user)) != TM_OK) { (void)hipFree(d_rows); return st; }
-  st = finish(g, 0, 1, count);
+  if ((st = launch(g, sl, 0, 1, true, tune, false, fn, user)) != TM_OK) { (void)hipFree(d_rows); return st; }
+  st = finish(g, sl, 0, 1, count);
   if (st == TM_OK) {
     int64_t n = std::min<int64_t>((int64_t)*count, cap);
     if (n > 0) {

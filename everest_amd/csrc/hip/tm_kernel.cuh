@@ -78,6 +78,13 @@ __device__ __forceinline__ bool in_first(const int (&r)[MAXN], int n, int node) 
   return hit;
 }
 
+// component i (uniform) of a register quadruple, kept in registers
+__device__ __forceinline__ unsigned pick4(uint4 v, int i) {
+  const unsigned a = (unsigned)opaque((int)v.x), b = (unsigned)opaque((int)v.y);
+  const unsigned c = (unsigned)opaque((int)v.z), d = (unsigned)opaque((int)v.w);
+  return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d));
+}
+
 __device__ __forceinline__ unsigned wave_min_u32(unsigned x) {
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) x = min(x, (unsigned)__shfl_xor((int)x, o));
@@ -134,6 +141,18 @@ __device__ __forceinline__ void st_rlx(unsigned long long *p, unsigned long long
 }
 __device__ __forceinline__ int add_rlx(int *p, int v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_rlx32(const unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rlx32(unsigned *p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Tag of the words of a donated record: launch generation (18 bits, never 0)
+// and lap of the record's ring slot (14 bits).
+__device__ __forceinline__ unsigned pool_tag(int gen, unsigned lap) {
+  return ((unsigned)gen << 14) | (lap & 0x3fffu);
 }
 
 // Motif-shape policy of the kernel.  DynSpec reads the per-query descriptor
@@ -329,8 +348,11 @@ __device__ __forceinline__ unsigned long long ref_probes(const tm_graph_view &g,
   return (unsigned long long)(e - lo) + (e < hi ? 1ull : 0ull);
 }
 
-template <int MAXE, int MAXN, bool ENUM, bool INSTR, class SP = DynSpec>
-__global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
+// WPE: wavefronts per SIMD the register allocator must leave room for (1: no
+// constraint, the compiler's choice; 6: at most 80 VGPRs, a few spills, 24
+// wavefronts per CU when LDS allows).
+template <int MAXE, int MAXN, bool ENUM, bool INSTR, class SP = DynSpec, int WPE = 1>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void mine_kernel(tm_kernel_args args) {
   extern __shared__ int lds[];
   const int lane = threadIdx.x;
   const int FR = args.frames;
@@ -359,20 +381,18 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   unsigned iter = 0;
   int qcur = -1;
   int top = 0;
-  int chunk_left = 0;      // 256-root blocks left in the current chunk
-  int next_root = 0;       // first root of the next block (uniform)
+  long long claim_next = 0, claim_end = 0;  // blocks of the current claim not yet started
+  long long qb0 = 0, qb1 = 0;               // blocks of the current query
   int blk = -1;            // first root of the block being dispatched (-1: none)
   int quarter = 0;         // quarter of the block: this lane's root is blk + 4 lane + quarter
-  int rk = -1;             // start rule whose queries are being checked
-  int ent = 0, ent_end = 0;// next / end qtab entry of that rule
-  unsigned smin = 0;       // smallest gap of the quarter's roots under rule rk (uniform)
-  unsigned lgap = 0;       // this lane's gap under rule rk
+  int cq = 0;              // query of the current chunk, and its table entry:
+  int cdelta = 0, cbeg = 0, cend = 0, crule = 0;
+  uint4 bgap = make_uint4(0u, 0u, 0u, 0u);  // gaps of this lane's 4 roots of the block
   bool roots_done = false; // the root chunks of every query are handed out
   const int home_stripe = blockIdx.x & (TM_STRIPES - 1);
   int stripe_off = 0;
   long long task_t0 = 0;   // wall clock when the current task (root group or donated frame) began
-  bool is_idle = false;    // counted in sched->idle (else in sched->busy)
-  if (lane == 0) add_rlx(&sched->busy, 1);
+  bool is_idle = false;    // gave up its busy credit (sched->busy starts at the grid size)
 
   // copy one frame between the LDS stack (slot w) and a pool record, one field per lane
   auto frame_field = [&](int f, int w) -> int * {
@@ -406,23 +426,29 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       // so busy == 0 means no work is left anywhere and a waiting
       // wavefront may leave.
       if (!is_idle) {
-        int nidle = 0;
-        if (lane == 0) { add_rlx(&sched->busy, -1); nidle = add_rlx(&sched->idle, 1) + 1; }
+        if (lane == 0) add_rlx(&sched->busy, -1);
         is_idle = true;
-        // enough wavefronts are already waiting for donations: leave
-        if (uni(__shfl(nidle, 0)) > args.idle_cap) { st_capexit++; break; }
       }
-      int ticket = 0;
-      if (lane == 0) ticket = (int)atomicAdd(&sched->pool_head, 1u);
-      ticket = uni(__shfl(ticket, 0));
-      if (ticket >= args.pool_cap) break;  // no record is ever written there
-      const unsigned long long *rec = args.pool + (size_t)ticket * TM_POOL_FIELDS;
+      {
+        // enough wavefronts already wait on tickets without records: leave
+        // (before drawing one: every drawn ticket is served or outlives the work)
+        int waiting = 0;
+        if (lane == 0) waiting = (int)ld_rlx(&sched->pool_head) - (int)ld_rlx(&sched->pool_tail);
+        if (uni(__shfl(waiting, 0)) >= args.idle_cap) { st_capexit++; break; }
+      }
+      unsigned ticket = 0;
+      if (lane == 0) ticket = atomicAdd(&sched->pool_head, 1u);
+      ticket = (unsigned)uni((int)__shfl((int)ticket, 0));
+      // record `ticket` lives in ring slot ticket mod pool_cap, lap ticket / pool_cap
+      const unsigned slot = ticket & (unsigned)(args.pool_cap - 1);
+      const unsigned tag = pool_tag(args.gen, ticket >> args.pool_shift);
+      const unsigned long long *rec = args.pool + (size_t)slot * TM_POOL_FIELDS;
       int fv = 0;
       bool got = false;
       for (unsigned spin = 0;; spin++) {
         st_polls++;
         unsigned long long word = lane < TM_POOL_FIELDS ? ld_rlx(rec + lane) : 0ull;
-        const bool mine = lane >= TM_POOL_FIELDS || (int)(word >> 32) == args.gen;
+        const bool mine = lane >= TM_POOL_FIELDS || (unsigned)(word >> 32) == tag;
         fv = (int)(unsigned)(word & 0xffffffffull);
         if (__ballot(!mine) == 0ull) { got = true; break; }
         if ((spin & 3u) == 3u) {
@@ -430,7 +456,11 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
           if (lane == 0) {
             b = ld_rlx(&sched->busy);
             ab = ld_rlx(&sched->abort);
-            if (args.limit_ticks > 0 && (long long)wall_clock64() - sched->t0 > args.limit_ticks) ab = 1;
+            if (!ab && args.limit_ticks > 0 && (long long)wall_clock64() - sched->t0 > args.limit_ticks) {
+              atomicOr(&sched->abort, 1);
+              atomicOr(&sched->error, 4);
+              ab = 1;
+            }
           }
           if (uni(__shfl(ab, 0))) break;
           if (uni(__shfl(b, 0)) == 0) break;  // every record is taken and nobody holds work
@@ -438,8 +468,9 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         __builtin_amdgcn_s_sleep(32);
       }
       if (!got) break;
+      // the frame is copied out below (fv): release the slot for its next lap
+      if (lane == 0) st_rlx32(args.pool_done + slot, tag);
       st_take++;
-      if (lane == 0) add_rlx(&sched->idle, -1);
       is_idle = false;  // the record's busy credit is now this wavefront's
       const int fq = uni(__shfl(fv, 0));
       if (fq != qcur) {
@@ -461,91 +492,75 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
     }
     if (top == 0) {
       // ---------------------------------------------------- root dispatch
-      // Roots are scanned in 256-root blocks over the union of the queries'
-      // root ranges; lane l holds roots blk + 4l .. blk + 4l + 3 and their
-      // gaps (one 16-byte load per start rule in use).  A block none of
-      // whose gaps is within the largest delta of its rule is skipped whole.
-      // Otherwise each quarter of the block (one root per lane) is checked
-      // against the queries of each rule in decreasing delta order, stopping
-      // at the first delta below the quarter's smallest gap, and the
-      // wavefront expands every query some lane passes, one at a time.
+      // The launch's roots form one space of 256-root blocks, query after
+      // query, handed out in fixed claims over striped counters.
+      // In a block, lane l holds roots blk + 4l .. blk + 4l + 3 and their
+      // gaps under the query's start rule (one 16-byte load).  A block none
+      // of whose gaps is within delta is skipped whole; otherwise its four
+      // quarters (one root per lane) are expanded in turn, each only if some
+      // lane's root passes.
       int qsel = -1;
       bool pass = false, inq = false;
       while (qsel < 0) {
         if (blk < 0) {
-          if (chunk_left == 0) {
-            // striped counters: stripe k hands out chunks [k C / S, (k+1) C / S);
-            // a wavefront starts on its own stripe and moves on when it runs dry
-            long long c = -1;
+          if (claim_next >= claim_end) {
+            // striped counters: stripe k hands out blocks [k B / S, (k+1) B / S)
+            // in claims of claim_blocks; a wavefront starts on its own stripe
+            // and moves on when it runs dry
+            bool got = false;
             while (stripe_off < TM_STRIPES) {
               const int k = (home_stripe + stripe_off) & (TM_STRIPES - 1);
-              const long long sb = args.total_chunks * k / TM_STRIPES;
-              const long long se = args.total_chunks * (k + 1) / TM_STRIPES;
+              const long long sb = args.total_blocks * k / TM_STRIPES;
+              const long long se = args.total_blocks * (k + 1) / TM_STRIPES;
+              const long long n = args.claim_blocks;
               long long o = 0;
-              if (lane == 0) o = (long long)atomicAdd(&sched->next_chunk[k][0], 1ull);
-              o = __shfl(o, 0);
-              if (sb + o < se) { c = sb + o; break; }
+              if (lane == 0) o = (long long)atomicAdd(&sched->next_chunk[k][0], (unsigned long long)n);
+              o = sb + __shfl(o, 0);
+              if (o < se) {
+                claim_next = o;
+                claim_end = min(o + n, se);
+                got = true;
+                break;
+              }
               stripe_off++;
             }
-            if (c < 0) { roots_done = true; break; }
-            const long long first = (long long)args.root_al + c * (long long)args.chunk_roots;
-            next_root = (int)first;
-            const long long left = ((long long)args.root_hi - first + TM_BLOCK - 1) / TM_BLOCK;
-            const int cap = args.chunk_roots / TM_BLOCK;
-            chunk_left = (int)(left < cap ? left : cap);
-            if (chunk_left <= 0) { chunk_left = 0; continue; }
+            if (!got) { roots_done = true; break; }
           }
-          blk = next_root;
-          next_root += TM_BLOCK;
-          chunk_left--;
-          const int r0 = blk + 4 * lane;
-          bool near = false;
-#pragma unroll
-          for (int k = 0; k < TM_NTFIRST; k++) {
-            if ((args.rule_mask >> k) & 1) {
-              uint4 x = *(const uint4 *)(g.rgap[k] + r0);
-              if (r0 < args.root_lo || r0 >= args.root_hi) x.x = ~0u;
-              if (r0 + 1 < args.root_lo || r0 + 1 >= args.root_hi) x.y = ~0u;
-              if (r0 + 2 < args.root_lo || r0 + 2 >= args.root_hi) x.z = ~0u;
-              if (r0 + 3 < args.root_lo || r0 + 3 >= args.root_hi) x.w = ~0u;
-              const unsigned mn = min(min(x.x, x.y), min(x.z, x.w));
-              near |= args.rule_maxdelta[k] >= 0 && mn <= (unsigned)args.rule_maxdelta[k];
+          const long long b = claim_next++;
+          if (b < qb0 || b >= qb1) {
+            // the block's query: the last q with block_prefix[q] <= b (uniform search)
+            int lo = 0, hi = args.nq;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (args.block_prefix[mid] <= b) lo = mid; else hi = mid;
             }
+            cq = lo;
+            qb0 = args.block_prefix[cq];
+            qb1 = args.block_prefix[cq + 1];
+            const int4 qt = *(const int4 *)(args.qtab + 4 * cq);
+            cdelta = qt.x; cbeg = qt.y; cend = qt.z; crule = qt.w;
           }
-          st_roots += (unsigned)(min(max(args.root_hi - blk, 0), TM_BLOCK) - min(max(args.root_lo - blk, 0), TM_BLOCK));
+          blk = (cbeg & ~3) + (int)(b - qb0) * TM_BLOCK;
+          if (ENUM && ld_rlx(args.queries[cq].stop)) { blk = -1; continue; }  // enumeration cap reached
+          const int r0 = blk + 4 * lane;
+          uint4 x = *(const uint4 *)(g.rgap[crule] + r0);
+          if (r0 < cbeg || r0 >= cend) x.x = ~0u;
+          if (r0 + 1 < cbeg || r0 + 1 >= cend) x.y = ~0u;
+          if (r0 + 2 < cbeg || r0 + 2 >= cend) x.z = ~0u;
+          if (r0 + 3 < cbeg || r0 + 3 >= cend) x.w = ~0u;
+          bgap = x;
+          st_roots += (unsigned)(min(max(cend - blk, 0), TM_BLOCK) - min(max(cbeg - blk, 0), TM_BLOCK));
+          const unsigned mn = min(min(x.x, x.y), min(x.z, x.w));
+          const bool near = cdelta >= 0 && mn <= (unsigned)cdelta;
           if (!INSTR && __ballot(near) == 0ull) { blk = -1; continue; }
-          quarter = 0;
-          rk = -1;
-          ent = ent_end = 0;
+          quarter = -1;
         }
-        if (ent >= ent_end) {
-          // next rule in use for this quarter, else the next quarter
-          int k = rk + 1;
-          while (k < TM_NTFIRST && !((args.rule_mask >> k) & 1)) k++;
-          if (k >= TM_NTFIRST) {
-            if (++quarter == 4) { blk = -1; continue; }
-            k = 0;
-            while (!((args.rule_mask >> k) & 1)) k++;
-          }
-          rk = k;
-          ent = args.rule_first[k];
-          ent_end = args.rule_end[k];
-          {
-            // the block was just read: this re-read of one quarter hits in cache
-            const int r = blk + 4 * lane + quarter;
-            lgap = (r >= args.root_lo && r < args.root_hi) ? g.rgap[rk][r] : ~0u;
-          }
-          smin = uni(wave_min_u32(lgap));
-          continue;
-        }
-        const int4 qt = *(const int4 *)(args.qtab + 4 * ent);  // (query, delta, root_beg, root_end)
-        if (!INSTR && (qt.y < 0 || smin > (unsigned)qt.y)) { ent = ent_end; continue; }
-        ent++;
-        if (ENUM && ld_rlx(args.queries[qt.x].stop)) continue;
+        if (++quarter == 4) { blk = -1; continue; }
         const int r = blk + 4 * lane + quarter;
-        inq = r >= args.root_lo && r < args.root_hi && r >= qt.z && r < qt.w;
-        pass = inq && qt.y >= 0 && lgap <= (unsigned)qt.y;
-        if (__ballot(pass) != 0ull || (INSTR && __ballot(inq) != 0ull)) qsel = qt.x;
+        const unsigned lg = pick4(bgap, quarter);
+        inq = r >= cbeg && r < cend;
+        pass = inq && cdelta >= 0 && lg <= (unsigned)cdelta;
+        if (__ballot(pass) != 0ull || (INSTR && __ballot(inq) != 0ull)) qsel = cq;
       }
       if (qsel < 0) continue;  // every root chunk is handed out
       if (qsel != qcur) {
@@ -772,7 +787,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
       if (lane == 0) {
         const int hd = (int)ld_rlx(&sched->pool_head);
         const int tl = (int)ld_rlx(&sched->pool_tail);
-        want = min(min(hd, args.pool_cap) - tl, args.steal_max);
+        want = min(hd - tl, args.steal_max);
       }
       want = uni(__shfl(want, 0));
       for (int d = 0; d < want; d++) {
@@ -781,17 +796,30 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
         const unsigned long long cm = __ballot(can);
         if (cm == 0ull) break;
         const int b = __ffsll((long long)cm) - 1;
-        int t = 0;
+        unsigned t = 0;
+        int ok = 1;
         if (lane == 0) {
           add_rlx(&sched->busy, 1);  // the record's credit, before it can be seen
-          t = (int)atomicAdd(&sched->pool_tail, 1u);
-          if (t >= args.pool_cap) add_rlx(&sched->busy, -1);
+          t = atomicAdd(&sched->pool_tail, 1u);
+          const unsigned lap = t >> args.pool_shift;
+          if (lap > 0) {
+            // the slot's previous lap must have been copied out by its taker
+            // (a resident wavefront holding that ticket, so this wait ends)
+            const unsigned prev = pool_tag(args.gen, lap - 1);
+            unsigned w = 0;
+            while (ld_rlx32(args.pool_done + (t & (unsigned)(args.pool_cap - 1))) != prev) {
+              if (ld_rlx(&sched->abort) || ++w > (1u << 24)) { ok = 0; break; }
+              __builtin_amdgcn_s_sleep(8);
+            }
+            if (!ok) atomicOr(&sched->error, 8);
+          }
         }
-        t = uni(__shfl(t, 0));
-        if (t >= args.pool_cap) break;
+        t = (unsigned)uni((int)__shfl((int)t, 0));
+        if (!uni(__shfl(ok, 0))) break;  // abort, or the error above ends the launch
         const int cur0 = f_cur[b], end0 = f_end[b];
         const int mid = cur0 + (end0 - cur0) / 2;
-        unsigned long long *rec = args.pool + (size_t)t * TM_POOL_FIELDS;
+        const unsigned tag = pool_tag(args.gen, t >> args.pool_shift);
+        unsigned long long *rec = args.pool + (size_t)(t & (unsigned)(args.pool_cap - 1)) * TM_POOL_FIELDS;
         if (lane < TM_POOL_FIELDS) {
           int v = 0;
           if (lane == 0) v = qcur;
@@ -800,7 +828,7 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
             int *src = frame_field(lane, b);
             v = src ? *src : 0;
           }
-          st_rlx(rec + lane, ((unsigned long long)(unsigned)args.gen << 32) | (unsigned)v);
+          st_rlx(rec + lane, ((unsigned long long)tag << 32) | (unsigned)v);
         }
         __syncthreads();
         if (lane == 0) f_end[b] = mid;
@@ -818,30 +846,31 @@ __global__ __launch_bounds__(64) void mine_kernel(tm_kernel_args args) {
   if (!ENUM)
     for (int q = lane; q < args.nq; q += WAVE)
       if (q_cnt[q]) atomicAdd(args.queries[q].count, q_cnt[q]);
-  if (lane == 0) {
-    if (is_idle) add_rlx(&sched->idle, -1);
-    else add_rlx(&sched->busy, -1);
-  }
-  // work counters: one atomic per counter per wavefront
-  unsigned long long w0 = st_items, w1 = st_roots, w2 = wave_sum64(st_leaves), w3 = st_push;
-  if (lane == 0) {
-    atomicAdd(&sched->stats[TM_STAT_STEPS], st_steps);
-    atomicAdd(&sched->stats[TM_STAT_ITEMS], w0);
-    atomicAdd(&sched->stats[TM_STAT_ROOTS], w1);
-    atomicAdd(&sched->stats[TM_STAT_LEAVES], w2);
-    atomicAdd(&sched->stats[TM_STAT_PUSHES], w3);
-    if (st_don) atomicAdd(&sched->stats[TM_STAT_DONATED], st_don);
-    if (st_take) atomicAdd(&sched->stats[TM_STAT_TAKEN], st_take);
-    if (st_capexit) atomicAdd(&sched->stats[TM_STAT_CAPEXIT], st_capexit);
-    atomicAdd(&sched->stats[TM_STAT_POLLS], st_polls);
-  }
-  if constexpr (INSTR) {
-    unsigned long long a = wave_sum64(rx1), b = wave_sum64(rx2), c = wave_sum64(rp);
-    if (lane == 0) {
-      atomicAdd(&sched->stats[TM_STAT_REF_X1], a);
-      atomicAdd(&sched->stats[TM_STAT_REF_X2], b);
-      atomicAdd(&sched->stats[TM_STAT_REF_P], c);
+  if (lane == 0 && !is_idle) add_rlx(&sched->busy, -1);  // left early holding work (error or deadline)
+  // work counters: plain stores into this wavefront's own slot, summed by a
+  // one-block kernel after the launch (no same-address atomics from every wave)
+  {
+    unsigned long long v = 0;
+    switch (lane) {
+      case TM_STAT_STEPS: v = st_steps; break;
+      case TM_STAT_ITEMS: v = st_items; break;
+      case TM_STAT_ROOTS: v = st_roots; break;
+      case TM_STAT_PUSHES: v = st_push; break;
+      case TM_STAT_DONATED: v = st_don; break;
+      case TM_STAT_TAKEN: v = st_take; break;
+      case TM_STAT_CAPEXIT: v = st_capexit; break;
+      case TM_STAT_POLLS: v = st_polls; break;
+      default: break;
     }
+    const unsigned long long leaves = wave_sum64(st_leaves);
+    if (lane == TM_STAT_LEAVES) v = leaves;
+    if constexpr (INSTR) {
+      const unsigned long long a = wave_sum64(rx1), b = wave_sum64(rx2), c = wave_sum64(rp);
+      if (lane == TM_STAT_REF_X1) v = a;
+      if (lane == TM_STAT_REF_X2) v = b;
+      if (lane == TM_STAT_REF_P) v = c;
+    }
+    if (lane < TM_NSTATS) args.wstats[(size_t)blockIdx.x * TM_NSTATS + lane] = v;
   }
 }
 

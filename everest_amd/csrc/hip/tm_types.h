@@ -91,10 +91,10 @@ typedef struct {
   int *stop;                  // set once cap rows are claimed: no more roots of this query
 } tm_query_dev;
 
-#define TM_STRIPES 8  // root-chunk counters, one 128-byte line each (one per XCD slot)
+#define TM_STRIPES 8  // root-block counters, one 128-byte line each (one per XCD slot)
 
 typedef struct {
-  unsigned long long next_chunk[TM_STRIPES][16];
+  unsigned long long next_chunk[TM_STRIPES][16];  // blocks claimed per stripe
   int error;        // bit 0: frame stack overflow, bit 1: empty frame, bit 2: deadline passed,
                     // bit 3: a donated frame never became ready
   int abort;        // set once the deadline passed: every wavefront stops
@@ -108,16 +108,18 @@ typedef struct {
   int pad1[31];
   int busy;         // wavefronts holding work + records reserved and not yet taken
   int pad2[31];
-  int idle;         // wavefronts waiting for donated work
-  int pad3[31];
 } tm_sched;
 
-// A frame donated by a busy wavefront to an idle one (global pool): fields
-// q, lvl, cur, end, p, pend, tl, m[TM_MAXN], e[TM_MAXE], each stored as a
-// 64-bit word (launch generation << 32 | value) with device-scope atomic
-// stores, so a taker knows a record is complete when every word carries the
-// current generation: no fences, no cache invalidation.
+// A frame donated by a busy wavefront to an idle one: fields q, lvl, cur,
+// end, p, pend, tl, m[TM_MAXN], e[TM_MAXE], each stored as a 64-bit word
+// (tag << 32 | value) with device-scope atomic stores, so a taker knows the
+// record is complete when every word carries the tag it expects: no fences,
+// no cache invalidation.  Records live in a ring of pool_cap slots; record r
+// goes to slot r mod pool_cap with tag (launch generation, lap r / pool_cap),
+// and its taker stores that tag into pool_done[slot] once the frame is
+// copied out, which lets the slot's next lap be written.
 #define TM_POOL_FIELDS 23
+#define TM_GEN_BITS 18  // launch generations cycle through 1 .. 2^18 - 1
 
 enum {
   TM_STAT_STEPS = 0,   // wavefront expansion steps
@@ -131,34 +133,39 @@ enum {
   TM_STAT_DONATED = 8, // frames donated to the pool
   TM_STAT_TAKEN = 9,   // donated frames taken
   TM_STAT_CAPEXIT = 10,// wavefronts that left because enough were waiting
-  TM_STAT_POLLS = 11   // pool polls by waiting wavefronts
+  TM_STAT_POLLS = 11,  // pool polls by waiting wavefronts
+  TM_STAT_GRID = 12,   // launch shape, filled in by the host: wavefronts,
+  TM_STAT_CLAIM = 13,  //   256-root blocks per claim,
+  TM_STAT_LDS = 14,    //   LDS bytes per wavefront,
+  TM_STAT_WPE = 15     //   register-cap variant (1: none, 6: 80 VGPRs, 0: caller's kernel)
 };
+#define TM_NSTATS 16  // per-wavefront counter slot (one 128-byte line)
 
 typedef struct {
   tm_graph_view g;
   const tm_query_dev *queries;
-  // Query table in dispatch order: sorted by the start rule of edge 1, then
-  // by decreasing delta; 4 ints per entry: (query, delta, root_beg, root_end).
+  // 4 ints per query: (delta, root_beg, root_end, start rule of edge 1)
   const int *qtab;
+  // The launch's roots as one space of 256-root blocks, query after query:
+  // query q owns blocks [block_prefix[q], block_prefix[q + 1]), its block i
+  // starting at root (root_beg & ~3) + 256 i (nq + 1 entries).
+  const long long *block_prefix;
   int nq;
-  int chunk_roots;            // roots per dispatched chunk (multiple of 256)
-  int root_lo, root_hi;       // union of the queries' root ranges
-  int root_al;                // root_lo rounded down to a multiple of 4 (first block)
-  int rule_mask;              // start rules used by some query
-  int rule_first[TM_NTFIRST]; // qtab entries [rule_first[k], rule_end[k]) have rule k
-  int rule_end[TM_NTFIRST];
-  int rule_maxdelta[TM_NTFIRST]; // largest delta of rule k (-1: no query)
-  long long total_chunks;
+  long long total_blocks;
+  int claim_blocks;           // blocks per claim (the same for every query of the launch)
   int frames;                 // LDS frame capacity per wavefront
   int scan_max;               // last-edge ranges up to this size are scanned
   long long limit_ticks;      // wall-clock budget of the launch (0 = none)
   tm_sched *sched;
+  unsigned long long *wstats; // TM_NSTATS counters per wavefront (workgroup)
   unsigned long long *pool;   // donated frames, TM_POOL_FIELDS tagged words each
-  int pool_cap;
-  int gen;                    // launch generation tag of pool words
+  unsigned *pool_done;        // per ring slot: tag of the last record copied out
+  int pool_cap;               // ring slots, a power of two
+  int pool_shift;             // log2(pool_cap)
+  int gen;                    // launch generation, 1 .. 2^TM_GEN_BITS - 1
   int steal_every;            // busy wavefronts look for idle ones every this many steps
   int steal_max;              // frames donated per look
-  int idle_cap;               // most wavefronts kept waiting for donations
+  int idle_cap;               // most wavefronts kept waiting on tickets without records
   int steal_min;              // a donated half holds at least this many candidates
   long long steal_after_ticks; // a task donates only after running this long (wall clock)
 } tm_kernel_args;

@@ -98,10 +98,17 @@ class KernelGen:
         self.tables = motif_tables(self.edges)
         self.enumerate = self.options.enumerate()
         self.enum_NUM = self.options.enumNum() if self.enumerate else -1
-        t = dict(waves_per_cu=0, chunk_roots=0, scan_max=0, steal_every=0, steal_max=0, steal_min=0,
-                 steal_after_us=0)
+        t = dict(waves_per_cu=0, claim_blocks=0, scan_max=0, steal_every=0, steal_max=0, steal_min=0,
+                 steal_after_us=0, waves_per_simd_hint=0)
         if tuning:
             t.update({k: int(v) for k, v in tuning.items() if k in t})
+        # register cap of the counting kernel: 80 VGPRs (6 wavefronts per SIMD)
+        # when the LDS stack of this motif leaves room for them (at most two
+        # stacked levels), else the compiler's allocation
+        if t["waves_per_simd_hint"] <= 0:
+            t["waves_per_simd_hint"] = 6 if len(self.edges) - 2 <= 2 else 1
+        t["wpe"] = 6 if t["waves_per_simd_hint"] >= 6 else 1
+        t["waves_per_simd_hint"] = t["wpe"]
         if not self.options.opts.get("warpBalancing", True):
             t["steal_every"] = 1 << 30  # no work donation between wavefronts
         self.tuning = t

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define TM_ABI_VERSION 1
+#define TM_ABI_VERSION 2
 #define TM_MAX_MOTIF_EDGES 8
 #define TM_MAX_MOTIF_NODES 8
 
@@ -58,16 +58,21 @@ typedef struct tm_host_graph {
   const int32_t *Euid;       /* num_edges: original line numbers */
 } tm_host_graph;
 
-/* Launch parameters; zero fields take the library defaults. */
+/* Launch parameters; zero fields take the library defaults (occupancy-limited
+ * waves up to 32 per CU, scan_max 16, no time
+ * limit, steal_every 16, steal_max 4, steal_min 64, steal_after_us 1000,
+ * claim_blocks: about 8 claims of one query's roots per wavefront). */
 typedef struct tm_tuning {
   int32_t waves_per_cu;  /* resident wavefronts per CU in the persistent grid */
-  int32_t chunk_roots;   /* roots per dispatched chunk, multiple of 64 */
+  int32_t claim_blocks;  /* 256-root blocks per claim of the root schedule */
   int32_t scan_max;      /* last-edge ranges up to this size are scanned, larger ones use the pair index */
   int32_t time_limit_ms; /* abort the launch after this wall time on the device (0 = no limit) */
   int32_t steal_every;   /* busy wavefronts look for idle ones every this many steps */
   int32_t steal_max;     /* frames a busy wavefront donates per look */
   int32_t steal_min;     /* a donated half-frame holds at least this many candidates */
   int32_t steal_after_us; /* a wavefront donates only after holding one task this long */
+  int32_t waves_per_simd_hint; /* counting kernel variant: 1 = compiler's register allocation,
+                                  6 = capped at 80 VGPRs (24 wavefronts per CU); 0 = pick by LDS */
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */
@@ -94,7 +99,9 @@ tm_status tm_graph_free(tm_graph *g);
  * items examined, [2] roots dispatched, [3] last-edge range computations,
  * [4] frames pushed, [5..7] reference work of an instrumented run (X1, X2, P),
  * [8] frames donated between wavefronts, [9] donated frames taken, [10]
- * wavefronts that left early, [11] pool polls.  Up to 16 entries.
+ * wavefronts that left early, [11] pool polls, and the launch shape: [12]
+ * wavefronts in the grid, [13] blocks per claim, [14] LDS bytes per
+ * wavefront, [15] register-cap variant.  Up to 16 entries.
  * No reference counterpart (instrumentation).
  */
 tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n);

@@ -115,67 +115,88 @@ def root_passes(g, md, delta, r):
     return delta >= 0 and g["rgap"][md["tmode"][1]][r] <= delta
 
 
-def batch_dispatch(g, queries, chunk_roots=1024, nwaves=3, instr=False):
-    """Model of the multi-query root dispatch of one launch.
+STRIPES = 8
+
+
+def claim_blocks(total_blocks, nwaves, per_claim, rng):
+    """Fixed-size claims over striped counters, as the kernel claims root
+    blocks: stripe k holds blocks [k B / S, (k+1) B / S); a wavefront claims
+    per_claim blocks from its own stripe first and moves on when it runs
+    dry.  Wavefronts take turns in a random order; returns the claims as
+    (first, end) block ranges."""
+    counters = [0] * STRIPES
+    waves = [dict(home=w % STRIPES, off=0) for w in range(nwaves)]
+    claims = []
+    live = list(range(nwaves))
+    while live:
+        wi = live[int(rng.integers(0, len(live)))]
+        w = waves[wi]
+        got = False
+        while w["off"] < STRIPES:
+            k = (w["home"] + w["off"]) % STRIPES
+            sb, se = total_blocks * k // STRIPES, total_blocks * (k + 1) // STRIPES
+            o = sb + counters[k]
+            counters[k] += per_claim
+            if o < se:
+                claims.append((o, min(o + per_claim, se)))
+                got = True
+                break
+            w["off"] += 1
+        if not got:
+            live.remove(wi)
+    return claims
+
+
+def batch_dispatch(g, queries, nwaves=5, per_claim=3, seed=0, instr=False):
+    """Model of the root dispatch of one launch over several queries.
 
     queries: list of (motif descriptor, delta, root_beg, root_end).  Mirrors
-    the host plan (dispatch order sorted by start rule then decreasing delta,
-    rule ranges, the union of root ranges aligned down to 4, chunking) and the
-    kernel's block / quarter / rule / entry walk, with wavefronts taking
-    chunks in turn.  Returns the list of (query, root) pairs a wavefront
-    expands with that lane passing (instr: every in-range lane)."""
+    the host plan (one space of 256-root blocks, query after query; query
+    q's block i starts at root (root_beg & ~3) + 256 i) and the kernel's
+    walk: blocks claimed in fixed-size claims, the block's query found
+    by a search over the block prefix, the block read as one aligned 16-byte
+    load of gaps per lane, skipped when no gap is within delta, its quarters
+    expanded when some lane passes.  Returns the (query, root) pairs
+    expanded with that lane passing (instr: every in-range lane)."""
+    rng = np.random.default_rng(seed)
     nq = len(queries)
-    root_lo = min(q[2] for q in queries)
-    root_hi = max(max(q[3] for q in queries), root_lo)
-    for md, _, _, _ in queries:
+    prefix = [0]
+    for md, d, b, e in queries:
         assert 0 <= md["tmode"][1] < 6
-    order = sorted(range(nq), key=lambda i: (queries[i][0]["tmode"][1], -queries[i][1]))
-    rule_first, rule_end, rule_maxdelta, mask = [0] * 6, [0] * 6, [-1] * 6, 0
-    qtab = []
-    for i, qi in enumerate(order):
-        k = queries[qi][0]["tmode"][1]
-        if not (mask >> k) & 1:
-            mask |= 1 << k
-            rule_first[k] = i
-            rule_maxdelta[k] = queries[qi][1]
-        rule_end[k] = i + 1
-        qtab.append((qi, queries[qi][1], queries[qi][2], queries[qi][3]))
-    root_al = root_lo & ~3
-    assert chunk_roots % TM_BLOCK == 0
-    total_chunks = (root_hi - root_al + chunk_roots - 1) // chunk_roots
+        prefix.append(prefix[-1] + ((e - (b & ~3) + TM_BLOCK - 1) // TM_BLOCK if e > b else 0))
+    claims = claim_blocks(prefix[-1], nwaves, per_claim, rng)
+    seen = sorted(x for c in claims for x in range(*c))
+    assert seen == list(range(prefix[-1])), "every block claimed exactly once"
     out = []
-    for c in range(total_chunks):
-        first = root_al + c * chunk_roots
-        nblk = min(chunk_roots // TM_BLOCK, (root_hi - first + TM_BLOCK - 1) // TM_BLOCK)
-        for b in range(nblk):
-            blk = first + b * TM_BLOCK
-            near = False
-            for k in range(6):
-                if not (mask >> k) & 1:
-                    continue
-                for lane in range(WAVE):
-                    r0 = blk + 4 * lane
-                    assert r0 % 4 == 0 and r0 + 3 < g["epad"]  # one aligned 16-byte load
-                    x = [g["rgap"][k][r0 + i] if root_lo <= r0 + i < root_hi else 0xFFFFFFFF for i in range(4)]
-                    near |= rule_maxdelta[k] >= 0 and min(x) <= rule_maxdelta[k]
+    for c0, c1 in claims:
+        for bi in range(c0, c1):
+            lo, hi = 0, nq
+            while hi - lo > 1:
+                mid = (lo + hi) >> 1
+                if prefix[mid] <= bi:
+                    lo = mid
+                else:
+                    hi = mid
+            q = lo
+            assert prefix[q] <= bi < prefix[q + 1]
+            md, d, b, e = queries[q]
+            rule = md["tmode"][1]
+            blk = (b & ~3) + (bi - prefix[q]) * TM_BLOCK
+            gaps = []
+            for lane in range(WAVE):
+                r0 = blk + 4 * lane
+                assert r0 % 4 == 0 and r0 + 3 < g["epad"]  # one aligned 16-byte load
+                gaps.append([g["rgap"][rule][r0 + i] if b <= r0 + i < e else 0xFFFFFFFF for i in range(4)])
+            near = any(d >= 0 and min(x) <= d for x in gaps)
             if not near and not instr:
                 continue
             for quarter in range(4):
-                for k in range(6):
-                    if not (mask >> k) & 1:
-                        continue
-                    rs = [blk + 4 * lane + quarter for lane in range(WAVE)]
-                    lg = [g["rgap"][k][r] if root_lo <= r < root_hi else 0xFFFFFFFF for r in rs]
-                    smin = min(lg)
-                    for e in range(rule_first[k], rule_end[k]):
-                        qid, dl, qb, qe = qtab[e]
-                        if not instr and (dl < 0 or smin > dl):
-                            break
-                        for lane, r in enumerate(rs):
-                            inq = root_lo <= r < root_hi and qb <= r < qe
-                            ps = inq and dl >= 0 and lg[lane] <= dl
-                            if ps or (instr and inq):
-                                out.append((qid, r))
+                for lane in range(WAVE):
+                    r = blk + 4 * lane + quarter
+                    inq = b <= r < e
+                    ps = inq and d >= 0 and gaps[lane][quarter] <= d
+                    if ps or (instr and inq):
+                        out.append((q, r))
     return out
 
 
@@ -504,10 +525,14 @@ class _Shared:
     def __init__(self, nchunks, pool_cap):
         self.next_chunk = [0] * STRIPES   # striped chunk counters
         self.nchunks = nchunks
-        self.pool = []          # donated frames, in reservation order
-        self.head = 0           # tickets drawn
-        self.taken = set()
+        assert pool_cap & (pool_cap - 1) == 0
         self.cap = pool_cap
+        self.ring = [None] * pool_cap   # slot -> (record index, frame)
+        self.done = [-1] * pool_cap     # slot -> last lap taken
+        self.head = 0           # tickets drawn
+        self.tail = 0           # records reserved
+        self.ndonated = 0
+        self.taken = set()
         self.busy = 0
         self.idle = 0
 
@@ -531,6 +556,7 @@ class Wave:
         self.roots_done = False
         self.is_idle = False
         self.ticket = None
+        self.pending = None
         self.steps = 0
         self.done = False
         self.home = Wave.count % STRIPES
@@ -539,6 +565,9 @@ class Wave:
         sh.busy += 1
 
     def step(self):
+        if self.pending is not None:  # spinning on a ring slot
+            self.write_record()
+            return
         g, md, sh = self.g, self.md, self.sh
         M = md["M"]
         last = M - 2
@@ -553,14 +582,14 @@ class Wave:
             if self.ticket is None:
                 self.ticket = sh.head
                 sh.head += 1
-                if self.ticket >= sh.cap:
-                    sh.idle -= 1
-                    self.done = True
                 return
-            if self.ticket < len(sh.pool):
+            slot, lap = self.ticket % sh.cap, self.ticket // sh.cap
+            if sh.ring[slot] is not None and sh.ring[slot][0] == self.ticket:
                 assert self.ticket not in sh.taken
                 sh.taken.add(self.ticket)
-                f = sh.pool[self.ticket]
+                f = sh.ring[slot][1]
+                assert sh.done[slot] == lap - 1
+                sh.done[slot] = lap  # the slot's next lap may be written
                 self.ticket = None
                 sh.idle -= 1
                 self.is_idle = False  # inherits the record's busy credit
@@ -688,28 +717,39 @@ class Wave:
         self.steps += 1
         # donation
         if self.top > 0 and self.steps % self.steal_every == 0:
-            want = min(min(sh.head, sh.cap) - len(sh.pool), self.steal_max)
+            want = min(sh.head - sh.tail, self.steal_max)
             for _ in range(max(0, want)):
                 cand = [i for i in range(min(self.top, WAVE)) if frames[i]["end"] - frames[i]["cur"] >= 2 * self.steal_min]
                 if not cand:
                     break
-                b = cand[0]
                 sh.busy += 1  # the record's credit, before it can be seen
-                t = len(sh.pool)
-                if t >= sh.cap:
-                    sh.busy -= 1
-                    break
-                f = dict(frames[b])
-                mid = f["cur"] + (f["end"] - f["cur"]) // 2
-                don = dict(f)
-                don["cur"] = mid
-                sh.pool.append(don)
-                f["end"] = mid
-                frames[b] = f
+                r = sh.tail
+                sh.tail += 1
+                self.pending = (r, cand[0])
+                if not self.write_record():
+                    return  # wait for the ring slot's previous lap to be taken
+
+    def write_record(self):
+        """Write the pending donation once its ring slot is free."""
+        sh, frames = self.sh, self.frames
+        r, b = self.pending
+        slot, lap = r % sh.cap, r // sh.cap
+        if lap > 0 and sh.done[slot] != lap - 1:
+            return False
+        f = dict(frames[b])
+        mid = f["cur"] + (f["end"] - f["cur"]) // 2
+        don = dict(f)
+        don["cur"] = mid
+        sh.ring[slot] = (r, don)
+        sh.ndonated += 1
+        f["end"] = mid
+        frames[b] = f
+        self.pending = None
+        return True
 
 
 def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, steal_every=1, steal_max=4,
-              pool_cap=1 << 30, seed=0):
+              pool_cap=1 << 20, seed=0):
     """Simulate nwaves wavefronts stepping in a random interleaving; returns
     (count, number of donated frames)."""
     rng = np.random.default_rng(seed)
@@ -724,6 +764,6 @@ def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, ste
         w.step()
         guard += 1
         assert guard < 10_000_000, "no progress"
-    assert len(sh.taken) == len(sh.pool), "a donated record was never taken"
+    assert len(sh.taken) == sh.tail == sh.ndonated, "a donated record was never taken"
     assert sh.busy == 0 and sh.idle == 0
-    return sum(w.cnt for w in waves), len(sh.pool)
+    return sum(w.cnt for w in waves), sh.ndonated

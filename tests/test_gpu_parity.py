@@ -130,3 +130,40 @@ def test_pl8m_root_subranges_and_timeouts():
         got = dg.count(mot, 86400, beg, end)
         assert got == want, (edges, got, want)
     dg.close()
+
+
+def test_pl8m_batch_classes_concurrent():
+    """A batch mixing every shape class (3..6 edges, up to 4 and up to 8
+    nodes) runs as concurrent per-class launches on several streams; each
+    count equals the single-query count, and a root-range sample of each
+    equals the oracle."""
+    abi = _abi()
+    from everest_amd import core
+    from everest_amd.gen import graphs, queries
+    U, V, T = graphs.make("pl8m")
+    el = core.data.EdgeListLoader.fromArrays("pl8m", U, V, T)
+    hg = core.data.GraphDataLoader(el, core.data.FeatureLoader(el, "")).createGraphData()
+    arrs = {k: hg.array(k) for k in ("Eg", "inEdgesV", "outEdgesV", "inEdgesR", "outEdgesR", "Euid")}
+    g = dict(E=hg.numEdges, Eg=arrs["Eg"].reshape(-1, 3), inV=arrs["inEdgesV"], outV=arrs["outEdgesV"],
+             inR=arrs["inEdgesR"], outR=arrs["outEdgesR"], Euid=arrs["Euid"])
+    dg = abi.DeviceGraph(arrs["Eg"], arrs["inEdgesV"], arrs["outEdgesV"], arrs["inEdgesR"], arrs["outEdgesR"],
+                         arrs["Euid"])
+    rng = np.random.default_rng(23)
+    qs = []
+    for i in range(40):
+        M = 3 + i % 4
+        edges = queries.random_motif(rng, M, max_nodes=4 if i % 5 else 6)
+        qs.append((queries.as_motif(edges), int(rng.choice([600, 1800, 3600, 7200]))))
+    say("pl8m batch of", len(qs), "queries, classes", sorted({(len(m), max(max(a, b) for a, b, _ in m) + 1)
+                                                               for m, _ in qs}))
+    got = dg.batch_count(qs)
+    single = []
+    for i, (m, d) in enumerate(qs):
+        say("pl8m single", i, m, d)
+        single.append(dg.count(m, d))
+    assert got == single
+    for i, (m, d) in enumerate(qs[:12]):
+        beg = int(rng.integers(0, hg.numEdges - 100_000))
+        say("pl8m sample", i, beg)
+        assert dg.count(m, d, beg, beg + 50_000) == oracle.count(g, m, d, beg, beg + 50_000, nthreads=8)
+    dg.close()

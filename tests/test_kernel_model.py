@@ -78,15 +78,16 @@ def test_model_work_donation_between_wavefronts():
         want = oracle.count(g, mot, delta, nthreads=1)
         got, nd = km.run_waves(gv, md, delta, 0, g["E"] - M + 1, nwaves=int(rng.integers(2, 6)),
                                steal_every=int(rng.integers(1, 4)), steal_max=int(rng.integers(1, 5)),
-                               pool_cap=int(rng.choice([2, 1 << 30])), seed=int(rng.integers(0, 1 << 30)))
+                               pool_cap=int(rng.choice([1, 2, 4, 1 << 20])), seed=int(rng.integers(0, 1 << 30)))
         assert got == want
         donated += nd
     assert donated > 0
 
 
 def test_model_batch_dispatch_covers_every_match_once():
-    """The multi-query root dispatch (blocks of 256 roots, gap filter, queries
-    sorted by start rule and decreasing delta) expands each (query, root) pair
+    """The multi-query root dispatch (one space of 256-root blocks, query
+    after query, in fixed-size claims; the gap filter) expands
+    each (query, root) pair
     at most once and never drops a root that has a match: per-query sums of
     single-root oracle counts over the dispatched roots equal the full counts."""
     rng = np.random.default_rng(31)
@@ -105,7 +106,8 @@ def test_model_batch_dispatch_covers_every_match_once():
             e = int(rng.integers(b, E + 1)) if rng.random() < 0.7 else E
             qs.append((km.build_motif([(a, c) for a, c, _ in mot]), d, b, e))
             mots.append(mot)
-        pairs = km.batch_dispatch(gv, qs, chunk_roots=int(rng.choice([256, 512, 1024])))
+        pairs = km.batch_dispatch(gv, qs, nwaves=int(rng.integers(1, 12)), per_claim=int(rng.integers(1, 6)),
+                                  seed=it)
         assert len(pairs) == len(set(pairs))
         for qi, (md, d, b, e) in enumerate(qs):
             roots = [r for (q, r) in pairs if q == qi]
@@ -113,6 +115,6 @@ def test_model_batch_dispatch_covers_every_match_once():
             got = sum(oracle.count(g, mots[qi], d, r, r + 1, nthreads=1) for r in roots)
             assert got == oracle.count(g, mots[qi], d, b, e, nthreads=1), (it, qi)
         # instrumented launches expand every in-range root of every query
-        ipairs = km.batch_dispatch(gv, qs, chunk_roots=256, instr=True)
+        ipairs = km.batch_dispatch(gv, qs, instr=True)
         for qi, (md, d, b, e) in enumerate(qs):
             assert sorted(r for (q, r) in ipairs if q == qi) == list(range(b, e))
