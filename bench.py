@@ -13,9 +13,10 @@ query.  Queries that exceed the per-query budget in the plain pass are listed
 and left out of both passes (reported in the JSON line).
 
 value = batched queries/s.  roofline: algorithmic bytes of the reference's own
-work for the same queries (12 R + 8 X1 + 16 X2 + 16 P, counted exactly by the
-instrumented kernel) / duration of the batched mining kernel (HIP events on the
-library stream).  cpu_baseline: the C oracle (a multithreaded restatement of the
+work (12 R + 8 X1 + 16 X2 + 16 P, counted exactly by the instrumented kernel)
+per launch / mean launch duration (HIP events on the library stream), for the
+single-query kernel instantiation with the largest total time in the plain
+pass.  cpu_baseline: the C oracle (a multithreaded restatement of the
 reference DFS) on a bounded root-range sample of the same queries.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -39,7 +40,7 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--graph", default="pl64m")
     ap.add_argument("--queries", type=int, default=256)
@@ -47,6 +48,9 @@ def main():
     ap.add_argument("--qlimit-ms", type=int, default=2000, help="per-query budget in the plain pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget (0 = skip)")
     ap.add_argument("--json-out", default="")
+    ap.add_argument("--plain-only", action="store_true", help="stop after the plain pass (counter runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also copy this many bytes device-to-device (HBM counter scale for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -75,23 +79,37 @@ def main():
     qs = queries.batch_queries(args.queries, seed=args.seed)
     motifs = [(queries.as_motif(e), d) for e, d in qs]
 
-    # ---- plain: one query per launch
-    dg.count(motifs[0][0], motifs[0][1])  # warm the kernels
-    plain_ms, plain_counts, kernel_ms, over = [], [], [], []
+    # ---- screen: queries over the per-query budget are left out of both
+    # passes.  Screening runs each query alone through the batch kernel (a
+    # different instantiation), so the single-query kernel's launches below
+    # are exactly the kept queries.
     lim = abi.tuning(time_limit_ms=args.qlimit_ms)
+    over = []
+    t = time.perf_counter()
     for i, (m, d) in enumerate(motifs):
-        t = time.perf_counter()
         try:
-            c = dg.count(m, d, tune=lim)
+            dg.batch_count([(m, d)], tune=lim)
         except abi.TMError as e:
             if e.status != abi.TM_ERR_TIMEOUT:
                 raise
             over.append(i)
             log("query %d over budget (%d ms): %s delta=%d" % (i, args.qlimit_ms, qs[i][0], d))
+        if (i + 1) % 64 == 0:
+            log("screen: %d/%d queries, %.1fs" % (i + 1, len(motifs), time.perf_counter() - t))
+
+    # ---- plain: one query per launch
+    plain_ms, plain_counts, kernel_ms, variant = [], [], [], {}
+    for i, (m, d) in enumerate(motifs):
+        if i in over:
             continue
+        t = time.perf_counter()
+        c = dg.count(m, d)
         plain_ms.append((time.perf_counter() - t) * 1e3)
         kernel_ms.append(dg.last_kernel_ms())
         plain_counts.append((i, c))
+        st = dg.last_stats()
+        nmap = 4 if max(max(a, b) for a, b, _ in m) + 1 <= 4 else 8
+        variant[i] = (nmap, st["wpe"])
         if (i + 1) % 32 == 0:
             log("plain: %d/%d queries, %.1fs so far" % (i + 1, len(motifs), sum(plain_ms) / 1e3))
     keep = [i for i, _ in plain_counts]
@@ -99,17 +117,52 @@ def main():
     plain_qps = len(keep) / plain_total_s
     log("plain: %d queries in %.3fs (%.1f q/s), %d over budget" % (len(keep), plain_total_s, plain_qps, len(over)))
 
+    if args.plain_only:
+        log("plain pass only (--plain-only): no reference work, batch or CPU baseline")
+        calibrate_copy(args.calibrate_copy)
+        return
+
     # ---- reference work of the kept queries (instrumented kernel, untimed)
     work = np.zeros(4, dtype=np.float64)
+    balg = {}
     t = time.perf_counter()
     for n, i in enumerate(keep):
         c, w = dg.count_work(motifs[i][0], motifs[i][1])
         assert c == dict(plain_counts)[i]
         work += np.array(w, dtype=np.float64)
+        balg[i] = 12.0 * w[0] + 8.0 * w[1] + 16.0 * w[2] + 16.0 * w[3]
         if (n + 1) % 32 == 0:
             log("reference work: %d/%d queries, %.1fs" % (n + 1, len(keep), time.perf_counter() - t))
     b_alg = 12 * work[0] + 8 * work[1] + 16 * work[2] + 16 * work[3]
     log("reference work: R=%d X1=%d X2=%d P=%d -> %.3f GB" % (*work, b_alg / 1e9))
+    # roofline of the dominant single-query kernel instantiation (largest
+    # total time): algorithmic bytes per launch / mean launch duration (HIP
+    # events on the library's stream), over that instantiation's launches
+    kms = dict(zip(keep, kernel_ms))
+    groups = {}
+    for i in keep:
+        groups.setdefault(variant[i], []).append(i)
+    dom = max(groups, key=lambda k: sum(kms[i] for i in groups[k]))
+    dom_n = len(groups[dom])
+    dom_bytes = sum(balg[i] for i in groups[dom]) / dom_n
+    dom_ms = sum(kms[i] for i in groups[dom]) / dom_n
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    # the instantiation's name as the profiler prints it: mine_kernel<MAXE, MAXN,
+    # ENUM, INSTR, spec, waves-per-SIMD register cap, multi-query>
+    kname = "void tmk::mine_kernel<8, %d, false, false, tmk::DynSpec, %d, false>(tm_kernel_args)" % dom
+    log("dominant kernel %s: %d launches, %.3f GB and %.3f ms per launch -> %.1f GB/s" % (
+        kname, dom_n, dom_bytes / 1e9, dom_ms, achieved))
+    # HBM bytes per launch of that kernel on this workload, from the committed
+    # counter summary (two rocprofv3 --pmc runs, scaled by a 1 GiB copy)
+    traffic, traffic_src = None, None
+    tf = os.path.join(ROOT, "profiles", "mine_kernel_pl64m_config4_hbm.json")
+    if os.path.exists(tf) and args.graph == "pl64m" and args.queries == 256 and args.seed == 4:
+        with open(tf) as f:
+            t = json.load(f)
+        if t.get("kernel") == kname and t.get("launches") == dom_n:
+            traffic = round(t["hbm_bytes_per_launch"], 1)
+            traffic_src = "profiles/mine_kernel_pl64m_config4_hbm.md (FETCH_SIZE x %.3f + WRITE_SIZE x %.3f)" % (
+                t["fetch_scale"], t["write_scale"])
 
     # ---- batched: all kept queries in one launch per step
     batch = [motifs[i] for i in keep]
@@ -127,8 +180,6 @@ def main():
         log("step %d: %.2f ms (kernel %.2f ms)" % (s, step_ms[-1], bkern_ms[-1]))
     ms_per_step = float(np.mean(step_ms))
     batch_qps = len(keep) / (ms_per_step / 1e3)
-    kern_avg = float(np.mean(bkern_ms))
-    achieved = b_alg / (kern_avg / 1e3) / 1e9
     matches = float(sum(want))
 
     # ---- CPU baseline: the oracle on a bounded sample
@@ -169,10 +220,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "mine_kernel (batched launch)",
-            "algorithmic_bytes_per_launch": b_alg,
-            "kernel_ms": round(kern_avg, 3),
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "kernel": kname,
+            "launches": dom_n,
+            "algorithmic_bytes_per_launch": round(dom_bytes, 1),
+            "mean_launch_ms": round(dom_ms, 4),
+            "algorithmic_bytes_all_kept_queries": b_alg,
         },
         "cpu_baseline": cpu,
     }
@@ -181,6 +235,21 @@ def main():
     if args.json_out:
         with open(args.json_out, "w") as f:
             f.write(line + "\n")
+
+
+def calibrate_copy(nbytes):
+    """One device-to-device copy of nbytes: under a counter run its
+    FETCH_SIZE / WRITE_SIZE give the scale of those counters."""
+    if nbytes <= 0:
+        return
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("calibration copy: %d bytes read, %d written" % (a.numel() * 4, b.numel() * 4))
 
 
 def cpu_baseline(hg, arrs, motifs, keep, counts, dg, budget_s):
@@ -206,7 +275,7 @@ def cpu_baseline(hg, arrs, motifs, keep, counts, dg, budget_s):
         c = oracle.count(g, m, d, beg, beg + span, nthreads=threads)
         dt = time.perf_counter() - t
         t_used += dt
-        gc = dg.count(m, d, beg, beg + span)
+        gc, _ = dg.count_work(m, d, beg, beg + span)  # the instrumented kernel: not the one timed above
         assert gc == c, "GPU/oracle mismatch on sample"
         checked += 1
         roots = E - len(m) + 1

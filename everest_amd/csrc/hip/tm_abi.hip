@@ -267,18 +267,18 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
   return TM_OK;
 }
 
-template <int MAXN, bool ENUM, bool INSTR, int WPE>
+template <int MAXN, bool ENUM, bool INSTR, int WPE, bool MULTI>
 static tm_status occupancy(int lds, int *blocks) {
   HIPTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks, tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE>, 64, lds));
+      blocks, tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE, MULTI>, 64, lds));
   return TM_OK;
 }
 
-template <int MAXN, bool ENUM, bool INSTR, int WPE>
+template <int MAXN, bool ENUM, bool INSTR, int WPE, bool MULTI>
 static tm_status launch_t(Slot &sl, const tm_kernel_args &a, int grid, int lds) {
   HIPTRY(hipEventRecord(sl.ev0, sl.stream));
-  hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE>), dim3(grid), dim3(64), lds,
-                     sl.stream, a);
+  hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE, MULTI>), dim3(grid), dim3(64),
+                     lds, sl.stream, a);
   HIPTRY(hipGetLastError());
   HIPTRY(hipEventRecord(sl.ev1, sl.stream));
   return TM_OK;
@@ -286,28 +286,32 @@ static tm_status launch_t(Slot &sl, const tm_kernel_args &a, int grid, int lds) 
 
 // Kernel variants: node map of 4 or 8, count / enumerate / instrumented
 // count; counting also comes with a register cap of 80 VGPRs (wpe 6) for
-// launches whose LDS stack leaves room for more than 20 wavefronts per CU.
+// launches whose LDS stack leaves room for more than 20 wavefronts per CU,
+// and in a single-query and a multi-query (batch) form.
 // OCC: report the occupancy in *blocks instead of launching.
-template <bool OCC, int MAXN, bool ENUM, bool INSTR, int WPE>
+template <bool OCC, int MAXN, bool ENUM, bool INSTR, int WPE, bool MULTI>
 static tm_status variant(Slot &sl, int lds, int *blocks, const tm_kernel_args *a, int grid) {
-  if (OCC) return occupancy<MAXN, ENUM, INSTR, WPE>(lds, blocks);
-  return launch_t<MAXN, ENUM, INSTR, WPE>(sl, *a, grid, lds);
+  if (OCC) return occupancy<MAXN, ENUM, INSTR, WPE, MULTI>(lds, blocks);
+  return launch_t<MAXN, ENUM, INSTR, WPE, MULTI>(sl, *a, grid, lds);
 }
 
 template <bool OCC, int MAXN>
-static tm_status dispatch_n(Slot &sl, bool enumerate, bool instr, int wpe, int lds, int *blocks,
+static tm_status dispatch_n(Slot &sl, bool enumerate, bool instr, int wpe, bool multi, int lds, int *blocks,
                             const tm_kernel_args *a, int grid) {
-  if (instr) return variant<OCC, MAXN, false, true, 1>(sl, lds, blocks, a, grid);
-  if (enumerate) return variant<OCC, MAXN, true, false, 1>(sl, lds, blocks, a, grid);
-  if (wpe == 6) return variant<OCC, MAXN, false, false, 6>(sl, lds, blocks, a, grid);
-  return variant<OCC, MAXN, false, false, 1>(sl, lds, blocks, a, grid);
+  if (instr) return variant<OCC, MAXN, false, true, 1, false>(sl, lds, blocks, a, grid);
+  if (enumerate) return variant<OCC, MAXN, true, false, 1, false>(sl, lds, blocks, a, grid);
+  if (multi)
+    return wpe == 6 ? variant<OCC, MAXN, false, false, 6, true>(sl, lds, blocks, a, grid)
+                    : variant<OCC, MAXN, false, false, 1, true>(sl, lds, blocks, a, grid);
+  return wpe == 6 ? variant<OCC, MAXN, false, false, 6, false>(sl, lds, blocks, a, grid)
+                  : variant<OCC, MAXN, false, false, 1, false>(sl, lds, blocks, a, grid);
 }
 
 template <bool OCC>
-static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wpe, int lds, int *blocks,
+static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wpe, bool multi, int lds, int *blocks,
                           const tm_kernel_args *a, int grid) {
-  return nmap == 4 ? dispatch_n<OCC, 4>(sl, enumerate, instr, wpe, lds, blocks, a, grid)
-                   : dispatch_n<OCC, 8>(sl, enumerate, instr, wpe, lds, blocks, a, grid);
+  return nmap == 4 ? dispatch_n<OCC, 4>(sl, enumerate, instr, wpe, multi, lds, blocks, a, grid)
+                   : dispatch_n<OCC, 8>(sl, enumerate, instr, wpe, multi, lds, blocks, a, grid);
 }
 
 // Plan and launch the mining kernel for staged queries [q0, q0 + nq) of
@@ -315,7 +319,8 @@ static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wp
 // occupancy, pick the chunk size over the union of the root ranges, upload,
 // reset, launch.
 tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
-                 tm_launch_fn fn = nullptr, void *user = nullptr) {
+                 bool multi, tm_launch_fn fn = nullptr, void *user = nullptr) {
+  if (nq > 1 && (enumerate || instr || fn || !multi)) return fail(TM_ERR_INTERNAL, "several queries need the batch kernel");
   Scratch &s = sl.s;
   int maxM = 2, maxN = 2;
   long long span = 0;  // roots over all queries (aligned down to 4 per query)
@@ -342,10 +347,10 @@ tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm
   } else {
     if (wpe == 0) {  // auto: the register cap pays off only where LDS is not the limit
       int b6 = 0;
-      if ((st = dispatch<true>(sl, nmap, enumerate, instr, 6, lds, &b6, nullptr, 0)) != TM_OK) return st;
+      if ((st = dispatch<true>(sl, nmap, enumerate, instr, 6, multi, lds, &b6, nullptr, 0)) != TM_OK) return st;
       wpe = b6 >= 24 ? 6 : 1;
     }
-    if ((st = dispatch<true>(sl, nmap, enumerate, instr, wpe, lds, &blocks, nullptr, 0)) != TM_OK) return st;
+    if ((st = dispatch<true>(sl, nmap, enumerate, instr, wpe, multi, lds, &blocks, nullptr, 0)) != TM_OK) return st;
   }
   int wpc = (tune && tune->waves_per_cu > 0) ? tune->waves_per_cu : 32;
   if (blocks > 0) wpc = std::min(wpc, blocks);
@@ -435,7 +440,7 @@ tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm
     HIPTRY(hipEventRecord(sl.ev0, sl.stream));
     if (fn(1, &a, grid, lds, sl.stream, nullptr, user) != 0) return fail(TM_ERR_HIP, "custom kernel launch failed");
     HIPTRY(hipEventRecord(sl.ev1, sl.stream));
-  } else if ((st = dispatch<false>(sl, nmap, enumerate, instr, wpe, lds, &blocks, &a, grid)) != TM_OK) {
+  } else if ((st = dispatch<false>(sl, nmap, enumerate, instr, wpe, multi, lds, &blocks, &a, grid)) != TM_OK) {
     return st;
   }
   hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, sl.stream, s.d_wstats, grid, s.d_sched);
@@ -553,9 +558,11 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
     if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, device) == hipSuccess && rate > 0)
       g->wall_rate_khz = rate;
   }
-  if ((st0 = reserve(g, g->slot[0], 1)) != TM_OK) {
-    tm_graph_free(g);
-    return st0;
+  for (Slot &sl : g->slot) {  // every slot's stream, scheduler state and pool, up front
+    if ((st0 = reserve(g, sl, 1)) != TM_OK) {
+      tm_graph_free(g);
+      return st0;
+    }
   }
 
   auto cleanup = [&](tm_status s) {
@@ -775,7 +782,7 @@ static tm_status count_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc,
   Slot &sl = g->slot[0];
   tm_status st = stage_queries(g, sl, 0, 1, nullptr, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
-  if ((st = launch(g, sl, 0, 1, false, tune, false, fn, user)) != TM_OK) return st;
+  if ((st = launch(g, sl, 0, 1, false, tune, false, false, fn, user)) != TM_OK) return st;
   return finish(g, sl, 0, 1, count);
 }
 
@@ -799,7 +806,7 @@ tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, con
   Slot &sl = g->slot[0];
   tm_status st = stage_queries(g, sl, 0, 1, nullptr, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
-  if ((st = launch(g, sl, 0, 1, false, tune, true)) != TM_OK) return st;
+  if ((st = launch(g, sl, 0, 1, false, tune, true, false)) != TM_OK) return st;
   if ((st = finish(g, sl, 0, 1, count)) != TM_OK) return st;
   const tm_query_dev &q = sl.s.h_queries[0];
   work[0] = (uint64_t)(q.root_end - q.root_beg);
@@ -850,7 +857,7 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
       Slot &sl = g->slot[k];
       st = stage_queries(g, sl, 0, gr.n, order.data() + gr.first, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas,
                          nullptr, nullptr);
-      if (st == TM_OK) st = launch(g, sl, 0, gr.n, false, tune, false);
+      if (st == TM_OK) st = launch(g, sl, 0, gr.n, false, tune, false, true);
       if (st == TM_OK) launched++;
     }
     for (size_t k = 0; k < launched; k++) {  // wait for every launched group, even after an error
@@ -896,7 +903,7 @@ static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, 
   HIPTRY(hipMalloc(&d_rows, row_bytes));
   s.h_queries[0].rows = d_rows;
   s.h_queries[0].cap = cap;
-  if ((st = launch(g, sl, 0, 1, true, tune, false, fn, user)) != TM_OK) { (void)hipFree(d_rows); return st; }
+  if ((st = launch(g, sl, 0, 1, true, tune, false, false, fn, user)) != TM_OK) { (void)hipFree(d_rows); return st; }
   st = finish(g, sl, 0, 1, count);
   if (st == TM_OK) {
     int64_t n = std::min<int64_t>((int64_t)*count, cap);

@@ -351,7 +351,9 @@ __device__ __forceinline__ unsigned long long ref_probes(const tm_graph_view &g,
 // WPE: wavefronts per SIMD the register allocator must leave room for (1: no
 // constraint, the compiler's choice; 6: at most 80 VGPRs, a few spills, 24
 // wavefronts per CU when LDS allows).
-template <int MAXE, int MAXN, bool ENUM, bool INSTR, class SP = DynSpec, int WPE = 1>
+// MULTI: the launch carries several queries (batches); a single-query launch
+// skips the search for the query of a root block.
+template <int MAXE, int MAXN, bool ENUM, bool INSTR, class SP = DynSpec, int WPE = 1, bool MULTI = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void mine_kernel(tm_kernel_args args) {
   extern __shared__ int lds[];
   const int lane = threadIdx.x;
@@ -529,7 +531,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           const long long b = claim_next++;
           if (b < qb0 || b >= qb1) {
             // the block's query: the last q with block_prefix[q] <= b (uniform search)
-            int lo = 0, hi = args.nq;
+            int lo = 0, hi = MULTI ? args.nq : 1;
             while (hi - lo > 1) {
               const int mid = (lo + hi) >> 1;
               if (args.block_prefix[mid] <= b) lo = mid; else hi = mid;
