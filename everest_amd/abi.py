@@ -40,6 +40,16 @@ class Tuning(ctypes.Structure):
                 ("waves_per_simd_hint", ctypes.c_int32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("arena_bytes", ctypes.c_int64),
+                ("tune", Tuning)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("batch", ctypes.c_int32), ("count", ctypes.c_uint64),
+                ("rows", ctypes.c_int64), ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -64,8 +74,17 @@ def lib():
         L.tm_last_stats.argtypes = [P, P, I32]
         L.tm_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.tm_count_work.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P, P]
+        L.tm_server_create.argtypes = [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
+        L.tm_server_submit.argtypes = [P, I32, P, P, I32, I64, ctypes.POINTER(ctypes.c_uint64)]
+        L.tm_server_wait.argtypes = [P, ctypes.c_uint64, I32, ctypes.POINTER(ServerResult)]
+        L.tm_server_take_rows.argtypes = [P, ctypes.c_uint64, P, I64, ctypes.POINTER(I64)]
+        L.tm_server_release.argtypes = [P, ctypes.c_uint64]
+        L.tm_server_stats.argtypes = [P, P, I32]
+        L.tm_server_destroy.argtypes = [P]
         for name in ("tm_graph_upload", "tm_graph_free", "tm_graph_info", "tm_count", "tm_enum",
-                     "tm_batch_count", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work"):
+                     "tm_batch_count", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work",
+                     "tm_server_create", "tm_server_submit", "tm_server_wait", "tm_server_take_rows",
+                     "tm_server_release", "tm_server_stats", "tm_server_destroy"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -192,6 +211,71 @@ class DeviceGraph:
         if getattr(self, "handle", None):
             lib().tm_graph_free(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """A serving loop over a DeviceGraph (tm_server_*): submit from any
+    thread, wait for results; launches are formed by continuous batching."""
+
+    def __init__(self, graph, slots=0, max_batch=0, arena_bytes=0, tune=None):
+        cfg = ServerConfig()
+        cfg.slots, cfg.max_batch, cfg.arena_bytes = slots, max_batch, arena_bytes
+        if tune is not None:
+            cfg.tune = tune
+        h = ctypes.c_void_p()
+        check(lib().tm_server_create(graph.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.graph = graph
+
+    def submit(self, edges, delta, enum_cap=-1):
+        src, dst = motif_arrays(edges)
+        t = ctypes.c_uint64()
+        check(lib().tm_server_submit(self.handle, len(edges), _p(src), _p(dst), int(delta), int(enum_cap),
+                                     ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        """dict(status, batch, count, rows, latency_ms, queue_ms); None on timeout."""
+        r = ServerResult()
+        st = lib().tm_server_wait(self.handle, ctypes.c_uint64(ticket), int(timeout_ms), ctypes.byref(r))
+        if st == TM_ERR_TIMEOUT:
+            return None
+        check(st)
+        return dict(status=r.status, batch=r.batch, count=int(r.count), rows=int(r.rows), latency_ms=r.latency_ms,
+                    queue_ms=r.queue_ms)
+
+    def take_rows(self, ticket, M, max_rows):
+        rows = np.zeros((max(int(max_rows), 1), M), dtype=np.int32)
+        n = ctypes.c_int64()
+        check(lib().tm_server_take_rows(self.handle, ctypes.c_uint64(ticket), _p(rows), int(max_rows),
+                                        ctypes.byref(n)))
+        return rows[:n.value].copy()
+
+    def release(self, ticket):
+        check(lib().tm_server_release(self.handle, ctypes.c_uint64(ticket)))
+
+    def stats(self):
+        st = np.zeros(6, dtype=np.uint64)
+        check(lib().tm_server_stats(self.handle, _p(st), 6))
+        return dict(launches=int(st[0]), count_served=int(st[1]), enum_served=int(st[2]), max_batch=int(st[3]),
+                    waiting=int(st[4]), running=int(st[5]))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tm_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
     def __del__(self):
         try:

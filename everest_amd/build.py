@@ -50,7 +50,8 @@ def ext_suffix():
 def build_hip():
     out = os.path.join(LIB, "libtm_hip.so")
     _run([HIPCC] + HIP_FLAGS + ["-shared", "-I" + INC, "-I" + HIP_SRC,
-                                os.path.join(HIP_SRC, "tm_abi.hip"), "-o", out])
+                                os.path.join(HIP_SRC, "tm_abi.hip"), os.path.join(HIP_SRC, "tm_server.hip"),
+                                "-pthread", "-o", out])
     return out
 
 

@@ -199,19 +199,6 @@ int ceil_log2(long long x) {
 
 // ----------------------------------------------------------------- graph
 
-struct tm_graph {
-  int device = 0;
-  int E = 0, V = 0;
-  int num_cus = 0;
-  long long wall_rate_khz = 100000;  // wall_clock64 ticks per millisecond
-  std::vector<void *> allocs;
-  int64_t bytes = 0;
-  tm_graph_view view{};
-  tmh::Slot slot[TM_SLOTS];  // slot 0 also carries the upload
-  int last_slot = 0;         // slot of the last finished launch (stats, kernel time)
-  std::mutex mtx;            // one API call at a time per graph handle
-};
-
 namespace tmh {
 
 tm_status set_error(tm_status s, const std::string &msg) { return fail(s, msg); }
@@ -319,7 +306,7 @@ static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wp
 // occupancy, pick the chunk size over the union of the root ranges, upload,
 // reset, launch.
 tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
-                 bool multi, tm_launch_fn fn = nullptr, void *user = nullptr) {
+                 bool multi, tm_launch_fn fn, void *user) {
   if (nq > 1 && (enumerate || instr || fn || !multi)) return fail(TM_ERR_INTERNAL, "several queries need the batch kernel");
   Scratch &s = sl.s;
   int maxM = 2, maxN = 2;
@@ -493,6 +480,63 @@ tm_status reserve(tm_graph *g, Slot &sl, int nq) {
   s.h_qtab.resize(4 * (size_t)cap);
   s.h_prefix.resize(cap + 1);
   return TM_OK;
+}
+
+// Stage queries idx[0..nq) (indexes into the caller's arrays) at staging
+// positions [q0, q0 + nq) of the slot.
+tm_status stage_queries(tm_graph *g, Slot &sl, int q0, int nq, const int *idx, const int32_t *num_edges,
+                               const int32_t *msrc, const int32_t *mdst, int stride, const int32_t *deltas,
+                               const int32_t *begs, const int32_t *ends) {
+  tm_status st;
+  if ((st = reserve(g, sl, q0 + nq)) != TM_OK) return st;
+  Scratch &s = sl.s;
+  for (int i = 0; i < nq; i++) {
+    const int q = idx ? idx[i] : i;
+    tm_query_dev &qd = s.h_queries[q0 + i];
+    memset(&qd, 0, sizeof(qd));
+    if ((st = build_motif(num_edges[q], msrc + (size_t)q * stride, mdst + (size_t)q * stride, &qd.m)) != TM_OK)
+      return st;
+    if (deltas[q] < 0) return fail(TM_ERR_ARG, "delta must be >= 0");
+    qd.delta = deltas[q];
+    int beg = begs ? begs[q] : 0;
+    int end = ends ? ends[q] : -1;
+    if (end < 0) end = g->E - qd.m.M + 1;
+    if (beg < 0) beg = 0;
+    if (end > g->E) end = g->E;
+    if (end < beg) end = beg;
+    qd.root_beg = beg;
+    qd.root_end = end;
+    qd.count = s.d_counts + q0 + i;
+    qd.stop = s.d_stops + q0 + i;
+  }
+  return TM_OK;
+}
+
+// Check a completed launch of the slot: its scheduler state hs (copied back)
+// gives the launch statistics and error bits.
+tm_status collect(tm_graph *g, Slot &sl, const tm_sched &hs) {
+  memcpy(sl.last_stats, hs.stats, sizeof(sl.last_stats));
+  sl.last_stats[TM_STAT_GRID] = (unsigned long long)sl.last_grid;
+  sl.last_stats[TM_STAT_CLAIM] = (unsigned long long)sl.last_claim;
+  sl.last_stats[TM_STAT_LDS] = (unsigned long long)sl.last_lds;
+  sl.last_stats[TM_STAT_WPE] = (unsigned long long)sl.last_wpe;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) sl.last_kernel_ms = ms;
+  g->last_slot = (int)(&sl - g->slot);
+  if (hs.error & 4) return fail(TM_ERR_TIMEOUT, "query exceeded its time limit; counts discarded");
+  if (hs.error) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(hs.error) + ")");
+  return TM_OK;
+}
+
+// Wait for the slot's launch of staged queries [q0, q0 + nq); copy their
+// counts out and check the launch's error bits.
+tm_status finish(tm_graph *g, Slot &sl, int q0, int nq, uint64_t *counts) {
+  Scratch &s = sl.s;
+  tm_sched hs;
+  HIPTRY(hipMemcpyAsync(&hs, s.d_sched, sizeof(hs), hipMemcpyDeviceToHost, sl.stream));
+  HIPTRY(hipMemcpyAsync(counts, s.d_counts + q0, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, sl.stream));
+  HIPTRY(hipStreamSynchronize(sl.stream));
+  return collect(g, sl, hs);
 }
 
 }  // namespace tmh
@@ -719,57 +763,6 @@ tm_status tm_graph_info(const tm_graph *g, int64_t *device_bytes, int32_t *num_e
   if (num_edges) *num_edges = g->E;
   if (num_vertices) *num_vertices = g->V;
   if (device) *device = g->device;
-  return TM_OK;
-}
-
-// Stage queries idx[0..nq) (indexes into the caller's arrays) at staging
-// positions [q0, q0 + nq) of the slot.
-static tm_status stage_queries(tm_graph *g, Slot &sl, int q0, int nq, const int *idx, const int32_t *num_edges,
-                               const int32_t *msrc, const int32_t *mdst, int stride, const int32_t *deltas,
-                               const int32_t *begs, const int32_t *ends) {
-  tm_status st;
-  if ((st = reserve(g, sl, q0 + nq)) != TM_OK) return st;
-  Scratch &s = sl.s;
-  for (int i = 0; i < nq; i++) {
-    const int q = idx ? idx[i] : i;
-    tm_query_dev &qd = s.h_queries[q0 + i];
-    memset(&qd, 0, sizeof(qd));
-    if ((st = build_motif(num_edges[q], msrc + (size_t)q * stride, mdst + (size_t)q * stride, &qd.m)) != TM_OK)
-      return st;
-    if (deltas[q] < 0) return fail(TM_ERR_ARG, "delta must be >= 0");
-    qd.delta = deltas[q];
-    int beg = begs ? begs[q] : 0;
-    int end = ends ? ends[q] : -1;
-    if (end < 0) end = g->E - qd.m.M + 1;
-    if (beg < 0) beg = 0;
-    if (end > g->E) end = g->E;
-    if (end < beg) end = beg;
-    qd.root_beg = beg;
-    qd.root_end = end;
-    qd.count = s.d_counts + q0 + i;
-    qd.stop = s.d_stops + q0 + i;
-  }
-  return TM_OK;
-}
-
-// Wait for the slot's launch of staged queries [q0, q0 + nq); copy their
-// counts out and check the launch's error bits.
-static tm_status finish(tm_graph *g, Slot &sl, int q0, int nq, uint64_t *counts) {
-  Scratch &s = sl.s;
-  tm_sched hs;
-  HIPTRY(hipMemcpyAsync(&hs, s.d_sched, sizeof(hs), hipMemcpyDeviceToHost, sl.stream));
-  HIPTRY(hipMemcpyAsync(counts, s.d_counts + q0, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, sl.stream));
-  HIPTRY(hipStreamSynchronize(sl.stream));
-  memcpy(sl.last_stats, hs.stats, sizeof(sl.last_stats));
-  sl.last_stats[TM_STAT_GRID] = (unsigned long long)sl.last_grid;
-  sl.last_stats[TM_STAT_CLAIM] = (unsigned long long)sl.last_claim;
-  sl.last_stats[TM_STAT_LDS] = (unsigned long long)sl.last_lds;
-  sl.last_stats[TM_STAT_WPE] = (unsigned long long)sl.last_wpe;
-  float ms = 0.f;
-  if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) sl.last_kernel_ms = ms;
-  g->last_slot = (int)(&sl - g->slot);
-  if (hs.error & 4) return fail(TM_ERR_TIMEOUT, "query exceeded its time limit; counts discarded");
-  if (hs.error) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(hs.error) + ")");
   return TM_OK;
 }
 

@@ -1,5 +1,6 @@
 // tm_internal.h -- host-side state shared by the library's translation units.
 #pragma once
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,5 +47,39 @@ struct Slot {
 
 tm_status set_error(tm_status s, const std::string &msg);
 tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_dev *md);
+
+}  // namespace tmh
+
+// A graph resident on one device, with its launch slots.
+struct tm_graph {
+  int device = 0;
+  int E = 0, V = 0;
+  int num_cus = 0;
+  long long wall_rate_khz = 100000;  // wall_clock64 ticks per millisecond
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+  tm_graph_view view{};
+  tmh::Slot slot[TM_SLOTS];  // slot 0 also carries the upload
+  int last_slot = 0;         // slot of the last finished launch (stats, kernel time)
+  std::mutex mtx;            // one API call at a time per graph handle
+};
+
+namespace tmh {
+
+// Create the slot's stream and reserve its staging space for nq queries.
+tm_status reserve(tm_graph *g, Slot &sl, int nq);
+// Stage queries idx[0..nq) of the caller's arrays at staging positions
+// [q0, q0 + nq) of the slot (motif descriptors, delta, root range).
+tm_status stage_queries(tm_graph *g, Slot &sl, int q0, int nq, const int *idx, const int32_t *num_edges,
+                        const int32_t *msrc, const int32_t *mdst, int stride, const int32_t *deltas,
+                        const int32_t *begs, const int32_t *ends);
+// Enqueue reset + mining kernel + statistics for staged queries [q0, q0 + nq)
+// on the slot's stream (multi: the batch kernel; fn: a caller's kernel).
+tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
+                 bool multi, tm_launch_fn fn = nullptr, void *user = nullptr);
+// After the slot's stream has drained: statistics and error bits of its launch.
+tm_status collect(tm_graph *g, Slot &sl, const tm_sched &hs);
+// Synchronous: wait for the slot's launch and copy out its counts.
+tm_status finish(tm_graph *g, Slot &sl, int q0, int nq, uint64_t *counts);
 
 }  // namespace tmh

@@ -180,16 +180,70 @@ tm_status tm_enum_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, co
                          int64_t *rows_written);
 
 /*
- * Count many motif queries on one graph in a single persistent launch.
- * Query q uses edges msrc/mdst[q * TM_MAX_MOTIF_EDGES ...], num_edges[q],
- * deltas[q] and roots [0, E - num_edges[q] + 1).  counts[q] receives the
- * exact count, equal to tm_count of the same query.  No reference
- * counterpart: the reference runs one generated worker per query
+ * Count many motif queries on one graph in one call.  Query q uses edges
+ * msrc/mdst[q * TM_MAX_MOTIF_EDGES ...], num_edges[q], deltas[q] and roots
+ * [0, E - num_edges[q] + 1).  counts[q] receives the exact count, equal to
+ * tm_count of the same query.  The queries are grouped by stack depth; each
+ * group is one persistent launch of the batch kernel, and the groups run
+ * concurrently on the graph's launch slots (TM_SLOTS streams).  No
+ * reference counterpart: the reference runs one generated worker per query
  * (system/batch-query.py:12-19).
  */
 tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges,
                          const int32_t *msrc, const int32_t *mdst, const int32_t *deltas,
                          const tm_tuning *tune, uint64_t *counts);
+
+/*
+ * Serving: a request loop over a resident graph.  No reference counterpart
+ * (the reference answers one YAML query per process run,
+ * system/tmine/frontend.py:138-143).
+ *
+ * Any host thread submits count or enumerate requests.  A dispatcher thread
+ * forms launches from whatever is queued whenever one of the server's launch
+ * slots is free (continuous batching): queued count requests of one
+ * stack-depth class go together into one batch launch (at most max_batch),
+ * oldest class first; an enumerate request runs alone and is admitted only
+ * once its row buffer (enum_cap * num_edges int32) fits in the enumeration
+ * arena.  Launches on different slots run concurrently on the device, so
+ * requests that arrive while others run overlap with them.  Each result is
+ * the same as tm_count / tm_enum of the request alone.  While a server
+ * exists, other calls on its graph wait.
+ */
+typedef struct tm_server tm_server;
+
+typedef struct tm_server_config {
+  int32_t slots;        /* launch slots used, 1 .. 4 (0: 4) */
+  int32_t max_batch;    /* count requests per launch, 1 .. 512 (0: 256) */
+  int64_t arena_bytes;  /* device memory for enumeration rows in flight (0: 1 GiB) */
+  tm_tuning tune;       /* launch parameters of every launch */
+} tm_server_config;
+
+typedef struct tm_server_result {
+  int32_t status;       /* TM_OK or the request's error */
+  int32_t batch;        /* requests in the launch that served it */
+  uint64_t count;       /* exact match count */
+  int64_t rows;         /* enumerated rows held for tm_server_rows */
+  double latency_ms;    /* submit -> result on the host */
+  double queue_ms;      /* submit -> launch */
+} tm_server_result;
+
+tm_status tm_server_create(tm_graph *g, const tm_server_config *cfg, tm_server **out);
+/* enum_cap < 0: count only; >= 0: enumerate up to enum_cap rows. */
+tm_status tm_server_submit(tm_server *s, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                           int32_t delta, int64_t enum_cap, uint64_t *ticket);
+/* Wait up to timeout_ms (< 0: no limit) for a request; TM_ERR_TIMEOUT if it
+ * is still running (nothing is lost: wait again). */
+tm_status tm_server_wait(tm_server *s, uint64_t ticket, int32_t timeout_ms, tm_server_result *res);
+/* Copy a finished request's rows (at most max_rows) and forget the request. */
+tm_status tm_server_take_rows(tm_server *s, uint64_t ticket, int32_t *rows, int64_t max_rows,
+                              int64_t *rows_written);
+/* Forget a finished request. */
+tm_status tm_server_release(tm_server *s, uint64_t ticket);
+/* [0] launches, [1] count requests served, [2] enumerate requests served,
+ * [3] largest batch, [4] requests waiting, [5] requests running. */
+tm_status tm_server_stats(tm_server *s, uint64_t *stats, int32_t n);
+/* Finish every submitted request, stop the dispatcher and free the server. */
+tm_status tm_server_destroy(tm_server *s);
 
 #ifdef __cplusplus
 }

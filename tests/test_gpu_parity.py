@@ -167,3 +167,71 @@ def test_pl8m_batch_classes_concurrent():
         say("pl8m sample", i, beg)
         assert dg.count(m, d, beg, beg + 50_000) == oracle.count(g, m, d, beg, beg + 50_000, nthreads=8)
     dg.close()
+
+
+def test_server_alone_and_in_mix():
+    """Serving: each request's result is the same alone and in a mix of
+    concurrently submitted count and enumerate requests (continuous batching
+    over several slots), and equals the single-query calls."""
+    abi = _abi()
+    import threading
+    rng = np.random.default_rng(41)
+    U, V, T = random_graph(rng, 3000, 60, 4000)
+    g = prep.preprocess(U, V, T)
+    dg = abi.DeviceGraph.from_dict(g)
+    reqs = []
+    for k in range(60):
+        mot = random_motif(rng, int(rng.integers(2, 7)))
+        d = int(rng.integers(0, 400))
+        cap = int(rng.integers(0, 5000)) if k % 5 == 0 else -1
+        reqs.append((mot, d, cap))
+    want = []  # (exact count, rows when the cap holds them all)
+    for mot, d, cap in reqs:
+        c = dg.count(mot, d)
+        want.append((c, dg.enum(mot, d, cap=max(c, 1))[1] if 0 <= cap and c <= cap else None))
+
+    def check(k, r, srv, t):
+        mot, d, cap = reqs[k]
+        total, rows_want = want[k]
+        assert r["status"] == 0, (k, r)
+        if cap < 0 or total <= cap:
+            assert r["count"] == total, (k, r, total)
+        else:  # the enumeration stops handing out roots once the cap is claimed
+            assert r["count"] >= cap, (k, r, cap)
+        if cap < 0:
+            srv.release(t)
+            return
+        rows = srv.take_rows(t, len(mot), cap)
+        if total <= cap:
+            assert sorted(map(tuple, rows.tolist())) == sorted(map(tuple, rows_want.tolist()))
+        else:
+            assert len(rows) == cap and len(set(map(tuple, rows.tolist()))) == cap
+
+    say("server: alone")
+    with abi.Server(dg, slots=4, max_batch=8) as srv:
+        for k, (mot, d, cap) in enumerate(reqs):
+            t = srv.submit(mot, d, cap)
+            check(k, srv.wait(t), srv, t)
+    say("server: mix from 3 threads")
+    with abi.Server(dg, slots=3, max_batch=8) as srv:
+        tickets = [None] * len(reqs)
+
+        def client(part):
+            for k in range(part, len(reqs), 3):
+                mot, d, cap = reqs[k]
+                tickets[k] = srv.submit(mot, d, cap)
+
+        th = [threading.Thread(target=client, args=(p,)) for p in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        batched = 0
+        for k in range(len(reqs)):
+            r = srv.wait(tickets[k])
+            batched = max(batched, r["batch"])
+            check(k, r, srv, tickets[k])
+        st = srv.stats()
+        say("server stats", st, "largest batch", batched)
+        assert st["count_served"] + st["enum_served"] == len(reqs)
+    dg.close()
