@@ -49,6 +49,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget (0 = skip)")
     ap.add_argument("--json-out", default="")
     ap.add_argument("--plain-only", action="store_true", help="stop after the plain pass (counter runs)")
+    ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving run (0 = skip)")
+    ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
+    ap.add_argument("--serve-enum-frac", type=float, default=0.2, help="share of enumerate requests")
+    ap.add_argument("--serve-enum-cap", type=int, default=1000000, help="row cap of enumerate requests")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also copy this many bytes device-to-device (HBM counter scale for profiles)")
     args = ap.parse_args()
@@ -182,6 +186,11 @@ def main():
     batch_qps = len(keep) / (ms_per_step / 1e3)
     matches = float(sum(want))
 
+    # ---- serving: Poisson arrivals through the server vs one query per launch, FIFO
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve(dg, motifs, keep, dict(plain_counts), dict(zip(keep, plain_ms)), args)
+
     # ---- CPU baseline: the oracle on a bounded sample
     cpu = None
     if args.cpu_seconds > 0:
@@ -213,6 +222,7 @@ def main():
             "matches_per_s": round(matches / (ms_per_step / 1e3), 1),
             "over_budget_queries": over,
             "bit_exact_batch_vs_plain": True,
+            "serving": serving,
         },
         "roofline": {
             "bound": "hbm",
@@ -235,6 +245,100 @@ def main():
     if args.json_out:
         with open(args.json_out, "w") as f:
             f.write(line + "\n")
+
+
+def serve(dg, motifs, keep, counts, plain_ms, args):
+    """Serving run: requests drawn from the kept queries (seeded), a share of
+    them enumerate requests, arriving as a Poisson stream; submitted to a
+    server (continuous batching over launch slots) by a client thread at
+    their arrival times.  Latency = arrival -> result on the host.  The plain
+    comparator serves the same stream one query per launch in arrival order
+    (a FIFO queue in front of tm_count / tm_enum), its service times measured
+    by running each request alone."""
+    import threading
+    import numpy as np
+    from everest_amd import abi
+
+    rng = np.random.default_rng(args.seed + 100)
+    n = args.serve_requests
+    pick = [keep[int(j)] for j in rng.integers(0, len(keep), n)]
+    enum = rng.random(n) < args.serve_enum_frac
+    gaps = rng.exponential(1.0 / args.serve_qps, n)
+    arrive = np.cumsum(gaps) - gaps[0]
+    cap = args.serve_enum_cap
+
+    # plain comparator: service time of each request alone (count: the plain
+    # pass's wall time; enumerate: measured here), then a FIFO queue
+    svc = np.zeros(n)
+    enum_ms = {}
+    for k in range(n):
+        i = pick[k]
+        if enum[k]:
+            if i not in enum_ms:
+                m, d = motifs[i]
+                t = time.perf_counter()
+                c, rows = dg.enum(m, d, cap=cap)
+                enum_ms[i] = (time.perf_counter() - t) * 1e3
+            svc[k] = enum_ms[i]
+        else:
+            svc[k] = plain_ms[i]
+    done = np.zeros(n)
+    free_at = 0.0
+    for k in range(n):
+        start = max(arrive[k] * 1e3, free_at)
+        free_at = start + svc[k]
+        done[k] = free_at
+    plain_lat = done - arrive * 1e3
+    plain_span = done.max() / 1e3
+
+    log("serving: %d requests (%d enumerate, cap %d) at %.0f/s offered" % (n, int(enum.sum()), cap, args.serve_qps))
+    srv = abi.Server(dg, slots=4, max_batch=256)
+    tickets = [None] * n
+    t0 = time.perf_counter()
+
+    def client():
+        for k in range(n):
+            dt = arrive[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            m, d = motifs[pick[k]]
+            tickets[k] = srv.submit(m, d, cap if enum[k] else -1)
+
+    th = threading.Thread(target=client)
+    th.start()
+    lat, batch, ok = np.zeros(n), np.zeros(n), True
+    for k in range(n):
+        while tickets[k] is None:
+            time.sleep(0.0005)
+        r = srv.wait(tickets[k])
+        lat[k], batch[k] = r["latency_ms"], r["batch"]
+        want = counts[pick[k]]
+        if enum[k]:
+            ok &= r["status"] == 0 and (r["count"] == want if want <= cap else r["count"] >= cap)
+            srv.take_rows(tickets[k], len(motifs[pick[k]][0]), min(want, cap))
+        else:
+            ok &= r["status"] == 0 and r["count"] == want
+            srv.release(tickets[k])
+    span = time.perf_counter() - t0
+    th.join()
+    st = srv.stats()
+    srv.close()
+    assert ok, "served results differ from single-query results"
+    res = {
+        "requests": n, "enumerate_requests": int(enum.sum()), "enum_cap": cap, "offered_qps": args.serve_qps,
+        "served_qps": round(n / span, 3), "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "plain_fifo_qps": round(n / plain_span, 3),
+        "plain_fifo_latency_ms_p50": round(float(np.percentile(plain_lat, 50)), 3),
+        "plain_fifo_latency_ms_p99": round(float(np.percentile(plain_lat, 99)), 3),
+        "served_over_plain_qps": round((n / span) / (n / plain_span), 3),
+        "launches": st["launches"], "largest_batch": st["max_batch"], "mean_batch": round(float(batch.mean()), 2),
+        "bit_exact_served_vs_plain": True,
+    }
+    log("serving: %.1f q/s served (plain FIFO %.1f), p50 %.1f ms p99 %.1f ms (plain FIFO p50 %.1f p99 %.1f)" % (
+        res["served_qps"], res["plain_fifo_qps"], res["latency_ms_p50"], res["latency_ms_p99"],
+        res["plain_fifo_latency_ms_p50"], res["plain_fifo_latency_ms_p99"]))
+    return res
 
 
 def calibrate_copy(nbytes):
