@@ -27,7 +27,7 @@
  * Pinned by: brute force from the match definition on random tiny graphs,
  * closed forms (parallel edges -> C(n,M), stars, cycles), and invariants
  * (root-range additivity, monotone in delta, relabel invariance) in
- * tests/test_oracle_*.py.
+ * tests/test_oracle.py.
  */
 #include <pthread.h>
 #include <stdint.h>
