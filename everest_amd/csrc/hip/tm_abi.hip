@@ -114,6 +114,21 @@ __global__ void tfirst_kernel(const tm_edge *__restrict__ Eg, const int *__restr
   t5[p] = (lo < outR[e.v + 1] && outPairNbr[lo] == e.u) ? T[outPairPos[lo]] : NONE;
 }
 
+// First-candidate times in the slot order of one candidate list.
+__global__ void stf_kernel(const int *__restrict__ pos, int E, const int *t0, const int *t1, const int *t2,
+                           const int *t3, const int *t4, const int *t5, int *o0, int *o1, int *o2, int *o3, int *o4,
+                           int *o5) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= E) return;
+  const int p = pos[s];
+  o0[s] = t0[p];
+  o1[s] = t1[p];
+  o2[s] = t2[p];
+  o3[s] = t3[p];
+  o4[s] = t4[p];
+  o5[s] = t5[p];
+}
+
 // Root gaps: first-candidate time minus the root's own time, per rule.
 __global__ void rgap_kernel(const tm_edge *__restrict__ Eg, int E, const int *t0, const int *t1, const int *t2,
                             const int *t3, const int *t4, const int *t5, unsigned *g0, unsigned *g1, unsigned *g2,
@@ -643,10 +658,9 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   for (int k = 0; k < TM_NTFIRST; k++) {
     ALLOC(vw.tfirst[k], e4);
   }
+  ALLOC(vw.stf, e4 * 4 * TM_NTFIRST);
   vw.epad = (int)(((long long)E + 255) / 256 * 256 + 256);
-  for (int k = 0; k < TM_NTFIRST; k++) {
-    ALLOC(vw.rgap[k], sizeof(unsigned) * (size_t)vw.epad);
-  }
+  ALLOC(vw.rgap, sizeof(unsigned) * (size_t)vw.epad * TM_NTFIRST);
   const int tmin = Eg[0].t, tmax = Eg[E - 1].t;
   const long long span = (long long)tmax - tmin + 1;
   int bshift = ceil_log2(std::max(1ll, (span * 16) / std::max(1, E)));
@@ -711,12 +725,24 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
                        vw.outPairSelf, (int *)vw.tfirst[0], (int *)vw.tfirst[1], (int *)vw.tfirst[2],
                        (int *)vw.tfirst[3], (int *)vw.tfirst[4], (int *)vw.tfirst[5]);
     TMPCHK(hipGetLastError());
-    for (int k = 0; k < TM_NTFIRST; k++)
-      TMPCHK(hipMemsetAsync((void *)vw.rgap[k], 0xff, sizeof(unsigned) * (size_t)vw.epad, s));
-    hipLaunchKernelGGL(rgap_kernel, gr, bl, 0, s, vw.Eg, E, vw.tfirst[0], vw.tfirst[1], vw.tfirst[2], vw.tfirst[3],
-                       vw.tfirst[4], vw.tfirst[5], (unsigned *)vw.rgap[0], (unsigned *)vw.rgap[1],
-                       (unsigned *)vw.rgap[2], (unsigned *)vw.rgap[3], (unsigned *)vw.rgap[4],
-                       (unsigned *)vw.rgap[5]);
+    {
+      const int *lists[4] = {vw.outPos, vw.inPos, vw.outPairPos, vw.inPairPos};  // TM_L_* order
+      for (int l = 0; l < 4; l++) {
+        int *o = (int *)vw.stf + (size_t)l * TM_NTFIRST * E;
+        hipLaunchKernelGGL(stf_kernel, gr, bl, 0, s, lists[l], E, vw.tfirst[0], vw.tfirst[1], vw.tfirst[2],
+                           vw.tfirst[3], vw.tfirst[4], vw.tfirst[5], o, o + (size_t)E, o + 2 * (size_t)E,
+                           o + 3 * (size_t)E, o + 4 * (size_t)E, o + 5 * (size_t)E);
+      }
+      TMPCHK(hipGetLastError());
+    }
+    TMPCHK(hipMemsetAsync((void *)vw.rgap, 0xff, sizeof(unsigned) * (size_t)vw.epad * TM_NTFIRST, s));
+    {
+      unsigned *rg = (unsigned *)vw.rgap;
+      const size_t ep = (size_t)vw.epad;
+      hipLaunchKernelGGL(rgap_kernel, gr, bl, 0, s, vw.Eg, E, vw.tfirst[0], vw.tfirst[1], vw.tfirst[2],
+                         vw.tfirst[3], vw.tfirst[4], vw.tfirst[5], rg, rg + ep, rg + 2 * ep, rg + 3 * ep,
+                         rg + 4 * ep, rg + 5 * ep);
+    }
     TMPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((nb + 1 + 255) / 256)), dim3(256), 0, s, vw.T, E, tmin,

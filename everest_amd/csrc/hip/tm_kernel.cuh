@@ -545,7 +545,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           blk = (cbeg & ~3) + (int)(b - qb0) * TM_BLOCK;
           if (ENUM && ld_rlx(args.queries[cq].stop)) { blk = -1; continue; }  // enumeration cap reached
           const int r0 = blk + 4 * lane;
-          uint4 x = *(const uint4 *)(g.rgap[crule] + r0);
+          uint4 x = *(const uint4 *)(g.rgap + (size_t)crule * (size_t)g.epad + r0);
           if (r0 < cbeg || r0 >= cend) x.x = ~0u;
           if (r0 + 1 < cbeg || r0 + 1 >= cend) x.y = ~0u;
           if (r0 + 2 < cbeg || r0 + 2 >= cend) x.z = ~0u;
@@ -700,10 +700,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 
     const int j = L + 1;  // motif edge of the candidates
     const int kind = SP::kind(md, j);
+    const int tmn = (j + 1 < M) ? SP::tmode(md, j + 1) : -1;  // pruning rule of edge j + 1
     bool valid = false;
-    int pos = 0;
+    int pos = 0, tfn = 0;
     if (has) {
       int node = 0;
+      // position, neighbour and the next edge's first-candidate time: three
+      // loads at the same slot of one list, issued together
+      const int lst = kind == TM_K_NEW_DST ? TM_L_OUT
+                      : (kind == TM_K_NEW_SRC ? TM_L_IN : (dir ? TM_L_INPAIR : TM_L_OUTPAIR));
+      if (tmn >= 0) tfn = g.stf[(size_t)(lst * TM_NTFIRST + tmn) * (size_t)g.E + slot];
       if (kind == TM_K_NEW_DST) {
         pos = g.outPos[slot]; node = g.outNbr[slot];
         valid = !in_first(m, SP::mapped(md, j), node);
@@ -738,8 +744,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           if ((long long)w + 1 >= q.cap) __hip_atomic_store(q.stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       } else {
-        const int tm = SP::tmode(md, j + 1);
-        if (valid && (tm < 0 || g.tfirst[tm][pos] <= tl)) {
+        if (valid && (tmn < 0 || tfn <= tl)) {
           const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
           cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max);
           st_leaves++;
@@ -750,10 +755,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       continue;
     }
     int cb = 0, ce = 0, cdir = 0;
-    {
-      const int tm = SP::tmode(md, j + 1);
-      if (valid && tm >= 0) valid = g.tfirst[tm][pos] <= tl;
-    }
+    if (valid && tmn >= 0) valid = tfn <= tl;
     if (valid) {
       const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
       cb = LL.lo;

@@ -67,18 +67,28 @@ typedef struct {
   const int *inNext;      // first slot of in(src of p) with position > p
   const int *outPairSelf; // slot of position p in the out-pair index of src of p
   const int *tfirst[TM_NTFIRST];  // TM_T_* arrays, one entry per position
+  // The same first-candidate times in the slot order of each candidate list
+  // (TM_L_*), one allocation: stf[(l * TM_NTFIRST + k) * E + s] =
+  // tfirst[k][position of slot s of list l], so the pruning test of a
+  // candidate is a coalesced load next to its position and neighbour instead
+  // of a gather (and the array is addressed by arithmetic, not by indexing
+  // a pointer table).
+  const int *stf;
   // Root gaps, one array per TM_T_* rule: tfirst - T[p] clamped to INT_MAX,
   // or 0xffffffff when p is a self-loop or has no candidate; a root passes a
   // query with that rule for edge 1 iff gap <= delta.  Padded with
   // 0xffffffff to a multiple of 256 entries plus 256, so a 256-root block
   // read as one 16-byte load per lane never leaves the array.
-  const unsigned *rgap[TM_NTFIRST];
+  const unsigned *rgap;    // TM_NTFIRST arrays of epad entries, one allocation
   int epad;               // entries per rgap array
   const int *bucket;      // bucket[k] = first position with T >= tmin + (k << bshift)
   int tmin, bshift;
   long long nbuckets;
   const int *Euid;        // original line number of each position
 } tm_graph_view;
+
+// Candidate lists, in the order of stf
+enum { TM_L_OUT = 0, TM_L_IN = 1, TM_L_OUTPAIR = 2, TM_L_INPAIR = 3 };
 
 typedef struct {
   tm_motif_dev m;

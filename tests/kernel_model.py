@@ -84,6 +84,10 @@ def build_view(g):
         rec = [outPairPos[k] for k in range(outR[v_], outR[v_ + 1]) if outPairNbr[k] == u_ and outPairPos[k] > p]
         if rec:
             tf[5, p] = T[min(rec)]
+    # first-candidate times in the slot order of each candidate list
+    # (tm_graph_view.stf, lists in TM_L_* order: out, in, out-pair, in-pair)
+    lists = [outPos, inPos, outPos[o], inPos[i]]
+    stf = [[Arr(tf[k][np.asarray(lst)], "stf%d_%d" % (l, k)) for k in range(6)] for l, lst in enumerate(lists)]
     # root gaps (tm_graph_view.rgap): first-candidate time minus the root's
     # time, clamped to INT_MAX; 0xffffffff for self-loops and no candidate;
     # padded to a multiple of 256 entries plus 256 with 0xffffffff
@@ -93,7 +97,7 @@ def build_view(g):
     d = np.minimum(tf - T[None, :].astype(np.int64), 2**31 - 1)
     rg[:, :E][ok] = d[ok]
     return dict(E=E, V=V, Eg=Eg, T=Arr(T, "T"), tfirst=[Arr(tf[k], "tfirst%d" % k) for k in range(6)],
-                rgap=[Arr(rg[k], "rgap%d" % k) for k in range(6)], epad=epad,
+                rgap=[Arr(rg[k], "rgap%d" % k) for k in range(6)], epad=epad, stf=stf,
                 outSelf=Arr(outSelf, "outSelf"), inSelf=Arr(inSelf, "inSelf"),
                 outNext=Arr(outNext, "outNext"), inNext=Arr(inNext, "inNext"),
                 outPairSelf=Arr(outPairSelf, "outPairSelf"),
@@ -107,6 +111,20 @@ def build_view(g):
 
 S_SEARCH, S_OUT_SELF, S_OUT_NEXT, S_IN_SELF, S_IN_NEXT, S_PAIR_SELF = range(6)
 TM_BLOCK = 4 * WAVE
+
+
+L_OUT, L_IN, L_OUTPAIR, L_INPAIR = range(4)
+
+
+def next_first_time(g, md, j, slot, direction, M):
+    """First-candidate time of motif edge j + 1 for the candidate at `slot`
+    of edge j's list, read from the slot-ordered copy (None: no rule)."""
+    tm = md["tmode"][j + 1] if j + 1 < M else -1
+    if tm < 0:
+        return None
+    kind = md["kind"][j]
+    lst = L_OUT if kind == K_NEW_DST else (L_IN if kind == K_NEW_SRC else (L_INPAIR if direction else L_OUTPAIR))
+    return g["stf"][lst][tm][slot]
 
 
 def root_passes(g, md, delta, r):
@@ -491,8 +509,10 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                         ref[2 if md["kind"][j + 1] == K_CLOSE else 1] += 1
                         ref[3] += ref_probes(g, md, j + 1, m, pos, f["pend"])
                     e = f["e"] + [pos]
-                    tm = md["tmode"][j + 1] if j + 1 < M else -1
-                    empty_next = tm >= 0 and g["tfirst"][tm][pos] > f["tl"]
+                    tfn = next_first_time(g, md, j, slot, f["dir"], M)
+                    if tfn is not None:
+                        assert tfn == g["tfirst"][md["tmode"][j + 1]][pos]
+                    empty_next = tfn is not None and tfn > f["tl"]
                     if j == last:
                         if enumerate_:
                             if cnt < cap:
@@ -699,8 +719,8 @@ class Wave:
                 valid = True
             if not valid:
                 continue
-            tm = md["tmode"][j + 1]
-            if tm >= 0 and g["tfirst"][tm][pos] > f["tl"]:
+            tfn = next_first_time(g, md, j, slot, f["dir"], M)
+            if tfn is not None and tfn > f["tl"]:
                 continue
             if j == last:
                 Lc = cand_start(g, md, j + 1, m, pos)
