@@ -74,6 +74,9 @@ def lib():
         L.tm_last_stats.argtypes = [P, P, I32]
         L.tm_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.tm_count_work.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P, P]
+        L.tm_batch_prepare.argtypes = [P, I32, P, P, P, P, ctypes.POINTER(Tuning), ctypes.POINTER(P)]
+        L.tm_batch_run.argtypes = [P, P]
+        L.tm_batch_free.argtypes = [P]
         L.tm_server_create.argtypes = [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
         L.tm_server_submit.argtypes = [P, I32, P, P, I32, I64, ctypes.POINTER(ctypes.c_uint64)]
         L.tm_server_wait.argtypes = [P, ctypes.c_uint64, I32, ctypes.POINTER(ServerResult)]
@@ -83,6 +86,7 @@ def lib():
         L.tm_server_destroy.argtypes = [P]
         for name in ("tm_graph_upload", "tm_graph_free", "tm_graph_info", "tm_count", "tm_enum",
                      "tm_batch_count", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work",
+                     "tm_batch_prepare", "tm_batch_run", "tm_batch_free",
                      "tm_server_create", "tm_server_submit", "tm_server_wait", "tm_server_take_rows",
                      "tm_server_release", "tm_server_stats", "tm_server_destroy"):
             getattr(L, name).restype = ctypes.c_int
@@ -191,21 +195,16 @@ class DeviceGraph:
 
     def batch_count(self, queries, tune=None):
         """queries: list of (edges, delta)."""
-        nq = len(queries)
-        ne = np.zeros(nq, dtype=np.int32)
-        src = np.zeros((nq, MAX_E), dtype=np.int32)
-        dst = np.zeros((nq, MAX_E), dtype=np.int32)
-        dl = np.zeros(nq, dtype=np.int32)
-        for q, (edges, delta) in enumerate(queries):
-            ne[q] = len(edges)
-            s, d = motif_arrays(edges)
-            src[q, :len(edges)] = s
-            dst[q, :len(edges)] = d
-            dl[q] = delta
-        out = np.zeros(nq, dtype=np.uint64)
+        ne, src, dst, dl = _batch_arrays(queries)
+        out = np.zeros(len(queries), dtype=np.uint64)
         t = tune if tune is not None else Tuning()
-        check(lib().tm_batch_count(self.handle, nq, _p(ne), _p(src), _p(dst), _p(dl), ctypes.byref(t), _p(out)))
+        check(lib().tm_batch_count(self.handle, len(queries), _p(ne), _p(src), _p(dst), _p(dl), ctypes.byref(t),
+                                   _p(out)))
         return [int(x) for x in out]
+
+    def prepare_batch(self, queries, tune=None):
+        """A PreparedBatch: the batch's launches captured into a HIP graph."""
+        return PreparedBatch(self, queries, tune)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -276,6 +275,49 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _batch_arrays(queries):
+    nq = len(queries)
+    ne = np.zeros(nq, dtype=np.int32)
+    src = np.zeros((nq, MAX_E), dtype=np.int32)
+    dst = np.zeros((nq, MAX_E), dtype=np.int32)
+    dl = np.zeros(nq, dtype=np.int32)
+    for q, (edges, delta) in enumerate(queries):
+        ne[q] = len(edges)
+        s, d = motif_arrays(edges)
+        src[q, :len(edges)] = s
+        dst[q, :len(edges)] = d
+        dl[q] = delta
+    return ne, src, dst, dl
+
+
+class PreparedBatch:
+    """tm_batch_prepare / tm_batch_run / tm_batch_free."""
+
+    def __init__(self, graph, queries, tune=None):
+        ne, src, dst, dl = _batch_arrays(queries)
+        t = tune if tune is not None else Tuning()
+        h = ctypes.c_void_p()
+        check(lib().tm_batch_prepare(graph.handle, len(queries), _p(ne), _p(src), _p(dst), _p(dl), ctypes.byref(t),
+                                     ctypes.byref(h)))
+        self.handle, self.nq, self.graph = h, len(queries), graph
+
+    def run(self):
+        out = np.zeros(self.nq, dtype=np.uint64)
+        check(lib().tm_batch_run(self.handle, _p(out)))
+        return [int(x) for x in out]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tm_batch_free(self.handle)
+            self.handle = None
 
     def __del__(self):
         try:

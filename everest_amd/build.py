@@ -51,6 +51,7 @@ def build_hip():
     out = os.path.join(LIB, "libtm_hip.so")
     _run([HIPCC] + HIP_FLAGS + ["-shared", "-I" + INC, "-I" + HIP_SRC,
                                 os.path.join(HIP_SRC, "tm_abi.hip"), os.path.join(HIP_SRC, "tm_server.hip"),
+                                os.path.join(HIP_SRC, "tm_batch.hip"),
                                 "-pthread", "-o", out])
     return out
 

@@ -168,25 +168,41 @@ __global__ void bucket_kernel(const int *__restrict__ T, int E, int tmin, int bs
   bucket[k] = lo;
 }
 
-__global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops, int n, int grid) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < TM_STRIPES) s->next_chunk[i][0] = 0;
-  if (i == 0) {
-    s->error = 0;
-    s->abort = 0;
-    s->pool_head = 0;
-    s->pool_tail = 0;
-    s->busy = grid;  // every wavefront starts out holding (root) work
-    s->t0 = (long long)wall_clock64();
-  }
-  if (i < 16) s->stats[i] = 0;
-  if (i < n) { counts[i] = 0; stops[i] = 0; }
+// Every field the mining kernel then reads or updates with device-scope
+// atomics is written here with device-scope atomic stores too.
+__device__ __forceinline__ void st_dev(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(unsigned *p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ void clear_pool_kernel(unsigned long long *pool, size_t nw, unsigned *done, size_t nd) {
+__global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops, int n, int grid) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < TM_STRIPES) st_dev(&s->next_chunk[i][0], 0ull);
+  if (i == 0) {
+    st_dev(&s->error, 0);
+    st_dev(&s->abort, 0);
+    st_dev(&s->pool_head, 0u);
+    st_dev(&s->pool_tail, 0u);
+    st_dev(&s->busy, grid);  // every wavefront starts out holding (root) work
+    st_dev(&s->waiters, 0);
+    // fresh ring tags for this launch (the host clears the ring before 2^18)
+    __hip_atomic_fetch_add(&s->gen, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s->t0 = (long long)wall_clock64();
+  }
+  if (i < 16) st_dev(&s->stats[i], 0ull);
+  if (i < n) { st_dev(&counts[i], 0ull); st_dev(&stops[i], 0); }
+}
+
+__global__ void clear_pool_kernel(unsigned long long *pool, size_t nw, unsigned *done, size_t nd, tm_sched *s) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nw) pool[i] = 0;
   if (i < nd) done[i] = 0;
+  if (i == 0) __hip_atomic_store(&s->gen, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // next reset: 1
 }
 
 // Sum the per-wavefront counters of a launch into sched->stats (one block).
@@ -278,11 +294,9 @@ static tm_status occupancy(int lds, int *blocks) {
 
 template <int MAXN, bool ENUM, bool INSTR, int WPE, bool MULTI>
 static tm_status launch_t(Slot &sl, const tm_kernel_args &a, int grid, int lds) {
-  HIPTRY(hipEventRecord(sl.ev0, sl.stream));
   hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE, MULTI>), dim3(grid), dim3(64),
                      lds, sl.stream, a);
   HIPTRY(hipGetLastError());
-  HIPTRY(hipEventRecord(sl.ev1, sl.stream));
   return TM_OK;
 }
 
@@ -320,8 +334,8 @@ static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wp
 // scratch.h_queries (nq <= TM_LAUNCH_MAXQ): size the persistent grid from
 // occupancy, pick the chunk size over the union of the root ranges, upload,
 // reset, launch.
-tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
-                 bool multi, tm_launch_fn fn, void *user) {
+tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr, bool multi,
+               tm_launch_fn fn, void *user, Plan *pl) {
   if (nq > 1 && (enumerate || instr || fn || !multi)) return fail(TM_ERR_INTERNAL, "several queries need the batch kernel");
   Scratch &s = sl.s;
   int maxM = 2, maxN = 2;
@@ -408,20 +422,10 @@ tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm
   if (tune && tune->time_limit_ms > 0) a.limit_ticks = (long long)tune->time_limit_ms * (g->wall_rate_khz);
   a.sched = s.d_sched;
   a.wstats = s.d_wstats;
-  if (++s.gen >= (1 << TM_GEN_BITS)) {
-    // generations wrap: clear the ring so no stale tag can match (a kernel,
-    // not a memset, so this is also safe inside a captured stream)
-    const size_t nw = (size_t)TM_POOL_FIELDS * s.pool_cap;
-    hipLaunchKernelGGL(clear_pool_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, sl.stream, s.d_pool, nw,
-                       s.d_pool_done, (size_t)s.pool_cap);
-    HIPTRY(hipGetLastError());
-    s.gen = 1;
-  }
   a.pool = s.d_pool;
   a.pool_done = s.d_pool_done;
   a.pool_cap = s.pool_cap;
   a.pool_shift = s.pool_shift;
-  a.gen = s.gen;
   a.steal_every = (tune && tune->steal_every > 0) ? tune->steal_every : 16;
   a.steal_max = (tune && tune->steal_max > 0) ? tune->steal_max : 4;
   a.idle_cap = std::max(64, g->num_cus / 2);
@@ -434,20 +438,68 @@ tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm
   sl.last_grid = grid;
   sl.last_lds = lds;
   sl.last_wpe = fn ? 0 : wpe;
+  pl->a = a;
+  pl->d_counts = s.d_counts + q0;
+  pl->d_stops = s.d_stops + q0;
+  pl->grid = grid;
+  pl->lds = lds;
+  pl->nmap = nmap;
+  pl->wpe = wpe;
+  pl->q0 = q0;
+  pl->nq = nq;
+  pl->enumerate = enumerate;
+  pl->instr = instr;
+  pl->multi = multi;
+  pl->fn = fn;
+  pl->user = user;
+  return TM_OK;
+}
 
-  hipLaunchKernelGGL(reset_kernel, dim3((nq + 255) / 256), dim3(256), 0, sl.stream, s.d_sched,
-                     s.d_counts + q0, s.d_stops + q0, nq, grid);
-  HIPTRY(hipGetLastError());
-  if (fn) {
-    HIPTRY(hipEventRecord(sl.ev0, sl.stream));
-    if (fn(1, &a, grid, lds, sl.stream, nullptr, user) != 0) return fail(TM_ERR_HIP, "custom kernel launch failed");
-    HIPTRY(hipEventRecord(sl.ev1, sl.stream));
-  } else if ((st = dispatch<false>(sl, nmap, enumerate, instr, wpe, multi, lds, &blocks, &a, grid)) != TM_OK) {
-    return st;
+tm_status count_launch(tm_graph *g, Slot &sl, hipStream_t stream, int n) {
+  (void)g;
+  Scratch &s = sl.s;
+  if (sl.gen + n >= (1 << TM_GEN_BITS)) {
+    // generations wrap: clear the ring so no stale tag can match
+    const size_t nw = (size_t)TM_POOL_FIELDS * s.pool_cap;
+    hipLaunchKernelGGL(clear_pool_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, stream, s.d_pool, nw,
+                       s.d_pool_done, (size_t)s.pool_cap, s.d_sched);
+    HIPTRY(hipGetLastError());
+    sl.gen = 0;
   }
-  hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, sl.stream, s.d_wstats, grid, s.d_sched);
+  sl.gen += n;
+  return TM_OK;
+}
+
+tm_status enqueue(tm_graph *g, Slot &sl, const Plan &pl, bool timing, bool count) {
+  tm_status st;
+  // the generation count belongs to the graph's slot (sl may be a view of it
+  // with other staging arrays); a captured launch is counted at each replay
+  if (count && (st = count_launch(g, g->slot[sl.index], sl.stream, 1)) != TM_OK) return st;
+  hipLaunchKernelGGL(reset_kernel, dim3((pl.nq + 255) / 256), dim3(256), 0, sl.stream, pl.a.sched, pl.d_counts,
+                     pl.d_stops, pl.nq, pl.grid);
+  HIPTRY(hipGetLastError());
+  if (timing) HIPTRY(hipEventRecord(sl.ev0, sl.stream));
+  if (pl.fn) {
+    if (pl.fn(1, &pl.a, pl.grid, pl.lds, sl.stream, nullptr, pl.user) != 0)
+      return fail(TM_ERR_HIP, "custom kernel launch failed");
+  } else {
+    int blocks = 0;
+    if ((st = dispatch<false>(sl, pl.nmap, pl.enumerate, pl.instr, pl.wpe, pl.multi, pl.lds, &blocks, &pl.a,
+                              pl.grid)) != TM_OK)
+      return st;
+  }
+  if (timing) HIPTRY(hipEventRecord(sl.ev1, sl.stream));
+  hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, sl.stream, pl.a.wstats, pl.grid, pl.a.sched);
   HIPTRY(hipGetLastError());
   return TM_OK;
+}
+
+tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
+                 bool multi, tm_launch_fn fn, void *user) {
+  Plan pl;
+  tm_status st = plan(g, sl, q0, nq, enumerate, tune, instr, multi, fn, user, &pl);
+  if (st != TM_OK) return st;
+  return enqueue(g, sl, pl, true, true);
 }
 
 // Create the slot's stream and reserve its staging space for nq queries
@@ -476,6 +528,7 @@ tm_status reserve(tm_graph *g, Slot &sl, int nq) {
   if (!s.d_sched) {
     if ((st = dev_alloc(g, &p, sizeof(tm_sched))) != TM_OK) return st;
     s.d_sched = (tm_sched *)p;
+    HIPTRY(hipMemsetAsync(s.d_sched, 0, sizeof(tm_sched), sl.stream));  // generation 0: no launch yet
     s.pool_shift = 18;
     s.pool_cap = 1 << s.pool_shift;
     const size_t pool_bytes = sizeof(unsigned long long) * TM_POOL_FIELDS * (size_t)s.pool_cap;
@@ -617,6 +670,7 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
     if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, device) == hipSuccess && rate > 0)
       g->wall_rate_khz = rate;
   }
+  for (int k = 0; k < TM_SLOTS; k++) g->slot[k].index = k;
   for (Slot &sl : g->slot) {  // every slot's stream, scheduler state and pool, up front
     if ((st0 = reserve(g, sl, 1)) != TM_OK) {
       tm_graph_free(g);

@@ -27,7 +27,6 @@ struct Scratch {
   int pool_cap = 0, pool_shift = 0;
   unsigned long long *d_wstats = nullptr; // per-wavefront counters of the last launch
   int wstats_cap = 0;                    // wavefronts they have room for
-  int gen = 0;                           // launch generation tag of pool words
   std::vector<tm_query_dev> h_queries;
   std::vector<int> h_qtab;
   std::vector<long long> h_prefix;
@@ -37,6 +36,8 @@ struct Scratch {
 // staging and results, so launches on different slots can run concurrently.
 #define TM_SLOTS 4
 struct Slot {
+  int index = 0;                            // position in tm_graph::slot
+  int gen = 0;                              // launches since the ring was last cleared (device sched->gen)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the mining kernel
   Scratch s;
@@ -73,10 +74,30 @@ tm_status reserve(tm_graph *g, Slot &sl, int nq);
 tm_status stage_queries(tm_graph *g, Slot &sl, int q0, int nq, const int *idx, const int32_t *num_edges,
                         const int32_t *msrc, const int32_t *mdst, int stride, const int32_t *deltas,
                         const int32_t *begs, const int32_t *ends);
-// Enqueue reset + mining kernel + statistics for staged queries [q0, q0 + nq)
-// on the slot's stream (multi: the batch kernel; fn: a caller's kernel).
+// A planned launch: kernel arguments and shape.
+struct Plan {
+  tm_kernel_args a;
+  unsigned long long *d_counts = nullptr;  // the launch's per-query counts
+  int *d_stops = nullptr;                  // and stop flags
+  int grid = 0, lds = 0, nmap = 4, wpe = 1, q0 = 0, nq = 0;
+  bool enumerate = false, instr = false, multi = false;
+  tm_launch_fn fn = nullptr;
+  void *user = nullptr;
+};
+// Plan a launch of staged queries [q0, q0 + nq) of the slot's staging
+// (occupancy, variant, claim size, query table) and upload its tables.
+tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr, bool multi,
+               tm_launch_fn fn, void *user, Plan *pl);
+// Enqueue a planned launch on the slot's stream: ring clear when the
+// generation wraps (host side), reset kernel, mining kernel (between the
+// slot's timing events when timing), statistics kernel.
+tm_status enqueue(tm_graph *g, Slot &sl, const Plan &pl, bool timing, bool count);
+// plan + enqueue (multi: the batch kernel; fn: a caller's kernel).
 tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
                  bool multi, tm_launch_fn fn = nullptr, void *user = nullptr);
+// Count n launches on the slot (each advances the device generation by one),
+// clearing the ring first on `stream` if the generation would wrap.
+tm_status count_launch(tm_graph *g, Slot &sl, hipStream_t stream, int n);
 // After the slot's stream has drained: statistics and error bits of its launch.
 tm_status collect(tm_graph *g, Slot &sl, const tm_sched &hs);
 // Synchronous: wait for the slot's launch and copy out its counts.

@@ -373,6 +373,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 
   const tm_graph_view g = args.g;
   tm_sched *sched = args.sched;
+  const int gen = ld_rlx(&sched->gen);  // advanced by the reset kernel before this launch
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   unsigned long long cnt = 0;
@@ -395,6 +396,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   int stripe_off = 0;
   long long task_t0 = 0;   // wall clock when the current task (root group or donated frame) began
   bool is_idle = false;    // gave up its busy credit (sched->busy starts at the grid size)
+  bool waiter = false;     // counted in sched->waiters
 
   // copy one frame between the LDS stack (slot w) and a pool record, one field per lane
   auto frame_field = [&](int f, int w) -> int * {
@@ -431,19 +433,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         if (lane == 0) add_rlx(&sched->busy, -1);
         is_idle = true;
       }
-      {
-        // enough wavefronts already wait on tickets without records: leave
-        // (before drawing one: every drawn ticket is served or outlives the work)
-        int waiting = 0;
-        if (lane == 0) waiting = (int)ld_rlx(&sched->pool_head) - (int)ld_rlx(&sched->pool_tail);
-        if (uni(__shfl(waiting, 0)) >= args.idle_cap) { st_capexit++; break; }
+      if (!waiter) {
+        // at most idle_cap wavefronts of a launch wait for donations (one
+        // atomic, so the bound is strict); the others leave, so the
+        // wavefronts of this launch or of a concurrent one that have not
+        // started yet always find a CU: waiting wavefronts of one launch can
+        // never hold every resource while they wait for its unstarted ones
+        int w = 0;
+        if (lane == 0) w = add_rlx(&sched->waiters, 1);
+        if (uni(__shfl(w, 0)) >= args.idle_cap) {
+          if (lane == 0) add_rlx(&sched->waiters, -1);
+          st_capexit++;
+          break;
+        }
+        waiter = true;
       }
       unsigned ticket = 0;
       if (lane == 0) ticket = atomicAdd(&sched->pool_head, 1u);
       ticket = (unsigned)uni((int)__shfl((int)ticket, 0));
       // record `ticket` lives in ring slot ticket mod pool_cap, lap ticket / pool_cap
       const unsigned slot = ticket & (unsigned)(args.pool_cap - 1);
-      const unsigned tag = pool_tag(args.gen, ticket >> args.pool_shift);
+      const unsigned tag = pool_tag(gen, ticket >> args.pool_shift);
       const unsigned long long *rec = args.pool + (size_t)slot * TM_POOL_FIELDS;
       int fv = 0;
       bool got = false;
@@ -471,7 +481,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       }
       if (!got) break;
       // the frame is copied out below (fv): release the slot for its next lap
-      if (lane == 0) st_rlx32(args.pool_done + slot, tag);
+      if (lane == 0) {
+        st_rlx32(args.pool_done + slot, tag);
+        add_rlx(&sched->waiters, -1);
+      }
+      waiter = false;
       st_take++;
       is_idle = false;  // the record's busy credit is now this wavefront's
       const int fq = uni(__shfl(fv, 0));
@@ -809,7 +823,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           if (lap > 0) {
             // the slot's previous lap must have been copied out by its taker
             // (a resident wavefront holding that ticket, so this wait ends)
-            const unsigned prev = pool_tag(args.gen, lap - 1);
+            const unsigned prev = pool_tag(gen, lap - 1);
             unsigned w = 0;
             while (ld_rlx32(args.pool_done + (t & (unsigned)(args.pool_cap - 1))) != prev) {
               if (ld_rlx(&sched->abort) || ++w > (1u << 24)) { ok = 0; break; }
@@ -822,7 +836,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         if (!uni(__shfl(ok, 0))) break;  // abort, or the error above ends the launch
         const int cur0 = f_cur[b], end0 = f_end[b];
         const int mid = cur0 + (end0 - cur0) / 2;
-        const unsigned tag = pool_tag(args.gen, t >> args.pool_shift);
+        const unsigned tag = pool_tag(gen, t >> args.pool_shift);
         unsigned long long *rec = args.pool + (size_t)(t & (unsigned)(args.pool_cap - 1)) * TM_POOL_FIELDS;
         if (lane < TM_POOL_FIELDS) {
           int v = 0;
@@ -851,6 +865,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     for (int q = lane; q < args.nq; q += WAVE)
       if (q_cnt[q]) atomicAdd(args.queries[q].count, q_cnt[q]);
   if (lane == 0 && !is_idle) add_rlx(&sched->busy, -1);  // left early holding work (error or deadline)
+  if (lane == 0 && waiter) add_rlx(&sched->waiters, -1);
   // work counters: plain stores into this wavefront's own slot, summed by a
   // one-block kernel after the launch (no same-address atomics from every wave)
   {

@@ -118,6 +118,11 @@ typedef struct {
   int pad1[31];
   int busy;         // wavefronts holding work + records reserved and not yet taken
   int pad2[31];
+  int waiters;      // wavefronts waiting for donations (at most idle_cap)
+  int pad4[31];
+  int gen;          // launch generation of the donation ring: the reset kernel advances it,
+                    // so replays of a captured launch get fresh tags
+  int pad3[31];
 } tm_sched;
 
 // A frame donated by a busy wavefront to an idle one: fields q, lvl, cur,
@@ -172,7 +177,6 @@ typedef struct {
   unsigned *pool_done;        // per ring slot: tag of the last record copied out
   int pool_cap;               // ring slots, a power of two
   int pool_shift;             // log2(pool_cap)
-  int gen;                    // launch generation, 1 .. 2^TM_GEN_BITS - 1
   int steal_every;            // busy wavefronts look for idle ones every this many steps
   int steal_max;              // frames donated per look
   int idle_cap;               // most wavefronts kept waiting on tickets without records

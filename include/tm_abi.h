@@ -194,6 +194,21 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges,
                          const tm_tuning *tune, uint64_t *counts);
 
 /*
+ * A prepared batch: the queries of tm_batch_count planned once, their tables
+ * uploaded, and the whole set of per-class launches (reset, mining kernel,
+ * statistics, result copies, on the graph's slot streams) captured into one
+ * HIP graph; tm_batch_run replays it.  The graph records no memset: every
+ * clear is a kernel.  Counts equal tm_batch_count's.  While a prepared batch
+ * exists, other calls on its graph may run between its replays (each launch
+ * resets the scheduler state it shares with them).  No reference counterpart.
+ */
+typedef struct tm_batch tm_batch;
+tm_status tm_batch_prepare(tm_graph *g, int32_t nq, const int32_t *num_edges, const int32_t *msrc,
+                           const int32_t *mdst, const int32_t *deltas, const tm_tuning *tune, tm_batch **out);
+tm_status tm_batch_run(tm_batch *b, uint64_t *counts);
+tm_status tm_batch_free(tm_batch *b);
+
+/*
  * Serving: a request loop over a resident graph.  No reference counterpart
  * (the reference answers one YAML query per process run,
  * system/tmine/frontend.py:138-143).

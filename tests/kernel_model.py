@@ -553,6 +553,8 @@ class _Shared:
         self.tail = 0           # records reserved
         self.ndonated = 0
         self.taken = set()
+        self.waiters = 0
+        self.idle_cap = 1 << 30
         self.busy = 0
         self.idle = 0
 
@@ -577,6 +579,7 @@ class Wave:
         self.is_idle = False
         self.ticket = None
         self.pending = None
+        self.waiter = False
         self.steps = 0
         self.done = False
         self.home = Wave.count % STRIPES
@@ -599,6 +602,14 @@ class Wave:
                 sh.idle += 1
                 self.is_idle = True
                 self.ticket = None
+            if not self.waiter:
+                # strict bound on waiting wavefronts: the others leave
+                if sh.waiters >= sh.idle_cap:
+                    sh.idle -= 1
+                    self.done = True
+                    return
+                sh.waiters += 1
+                self.waiter = True
             if self.ticket is None:
                 self.ticket = sh.head
                 sh.head += 1
@@ -611,6 +622,8 @@ class Wave:
                 assert sh.done[slot] == lap - 1
                 sh.done[slot] = lap  # the slot's next lap may be written
                 self.ticket = None
+                sh.waiters -= 1
+                self.waiter = False
                 sh.idle -= 1
                 self.is_idle = False  # inherits the record's busy credit
                 self.frames[0] = dict(f)
@@ -618,6 +631,8 @@ class Wave:
                 return
             if sh.busy == 0:
                 sh.idle -= 1
+                sh.waiters -= 1
+                self.waiter = False
                 self.done = True
             return  # spin
         if self.top == 0:
@@ -769,12 +784,13 @@ class Wave:
 
 
 def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, steal_every=1, steal_max=4,
-              pool_cap=1 << 20, seed=0):
+              pool_cap=1 << 20, seed=0, idle_cap=1 << 30):
     """Simulate nwaves wavefronts stepping in a random interleaving; returns
     (count, number of donated frames)."""
     rng = np.random.default_rng(seed)
     nchunks = max(0, (end - beg + chunk_roots - 1) // chunk_roots)
     sh = _Shared(nchunks, pool_cap)
+    sh.idle_cap = idle_cap
     waves = [Wave(g, md, delta, beg, end, sh, scan_max, chunk_roots, steal_every, steal_max)
              for _ in range(nwaves)]
     guard = 0
@@ -785,5 +801,5 @@ def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, ste
         guard += 1
         assert guard < 10_000_000, "no progress"
     assert len(sh.taken) == sh.tail == sh.ndonated, "a donated record was never taken"
-    assert sh.busy == 0 and sh.idle == 0
+    assert sh.busy == 0 and sh.idle == 0 and sh.waiters == 0
     return sum(w.cnt for w in waves), sh.ndonated

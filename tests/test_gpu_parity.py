@@ -235,3 +235,32 @@ def test_server_alone_and_in_mix():
         say("server stats", st, "largest batch", batched)
         assert st["count_served"] + st["enum_served"] == len(reqs)
     dg.close()
+
+
+@pytest.mark.skip(reason="graph-captured batches are switched off: on the MI355X two replays passed but an eager "
+                         "tm_batch_count after them did not finish; cause not established (tm_batch.hip)")
+def test_prepared_batch_graph_replays():
+    """A prepared batch (its launches captured into one HIP graph) replayed
+    several times, with eager single-query and batch calls on the same slots
+    in between: every replay equals the eager counts."""
+    abi = _abi()
+    from everest_amd.gen import graphs
+    U, V, T = graphs.make("zipf100k")
+    g = prep.preprocess(U, V, T)
+    dg = abi.DeviceGraph.from_dict(g)
+    rng = np.random.default_rng(43)
+    qs = [(random_motif(rng, int(rng.integers(2, 7))), int(rng.integers(600, 20000))) for _ in range(50)]
+    want = dg.batch_count(qs)
+    say("prepared batch of", len(qs))
+    pb = dg.prepare_batch(qs)
+    for rep in range(4):
+        got = pb.run()
+        say("replay", rep)
+        assert got == want, rep
+        # eager work on the same slots between replays
+        m, d = qs[rep]
+        assert dg.count(m, d) == want[rep]
+        assert dg.batch_count(qs[:7]) == want[:7]
+    pb.close()
+    assert [oracle.count(g, m, d, nthreads=8) for (m, d) in qs[:10]] == want[:10]
+    dg.close()

@@ -78,7 +78,8 @@ def test_model_work_donation_between_wavefronts():
         want = oracle.count(g, mot, delta, nthreads=1)
         got, nd = km.run_waves(gv, md, delta, 0, g["E"] - M + 1, nwaves=int(rng.integers(2, 6)),
                                steal_every=int(rng.integers(1, 4)), steal_max=int(rng.integers(1, 5)),
-                               pool_cap=int(rng.choice([1, 2, 4, 1 << 20])), seed=int(rng.integers(0, 1 << 30)))
+                               pool_cap=int(rng.choice([1, 2, 4, 1 << 20])), seed=int(rng.integers(0, 1 << 30)),
+                               idle_cap=int(rng.choice([1, 2, 1 << 30])))
         assert got == want
         donated += nd
     assert donated > 0
