@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget (0 = skip)")
     ap.add_argument("--json-out", default="")
     ap.add_argument("--plain-only", action="store_true", help="stop after the plain pass (counter runs)")
+    ap.add_argument("--over", default="", help="comma-separated over-budget query indexes: skips the screening pass")
+    ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned table (library defaults)")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
     ap.add_argument("--serve-enum-frac", type=float, default=0.2, help="share of enumerate requests")
@@ -83,14 +85,25 @@ def main():
     qs = queries.batch_queries(args.queries, seed=args.seed)
     motifs = [(queries.as_motif(e), d) for e, d in qs]
 
+    # launch parameters per shape class from the autotuner's table for this
+    # graph (python -m everest_amd.autotune), used by every pass below
+    tuned = os.path.join(ROOT, "everest_amd", "tuned", "%s.json" % args.graph)
+    tuning_src = "library defaults"
+    if os.path.exists(tuned) and not args.no_tuning:
+        from everest_amd import autotune
+        table, meta = autotune.load(tuned)
+        autotune.apply(dg, table)
+        tuning_src = "autotuned table everest_amd/tuned/%s.json (%d classes)" % (args.graph, len(table))
+    log("launch parameters: %s" % tuning_src)
+
     # ---- screen: queries over the per-query budget are left out of both
     # passes.  Screening runs each query alone through the batch kernel (a
     # different instantiation), so the single-query kernel's launches below
     # are exactly the kept queries.
     lim = abi.tuning(time_limit_ms=args.qlimit_ms)
-    over = []
+    over = [int(x) for x in args.over.split(",") if x.strip()]
     t = time.perf_counter()
-    for i, (m, d) in enumerate(motifs):
+    for i, (m, d) in enumerate(motifs if not args.over else []):
         try:
             dg.batch_count([(m, d)], tune=lim)
         except abi.TMError as e:
@@ -222,6 +235,7 @@ def main():
             "matches_per_s": round(matches / (ms_per_step / 1e3), 1),
             "over_budget_queries": over,
             "bit_exact_batch_vs_plain": True,
+            "launch_parameters": tuning_src,
             "serving": serving,
         },
         "roofline": {

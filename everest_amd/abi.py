@@ -74,6 +74,8 @@ def lib():
         L.tm_last_stats.argtypes = [P, P, I32]
         L.tm_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.tm_count_work.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P, P]
+        L.tm_shape_class.argtypes = [I32, P, P, ctypes.POINTER(I32)]
+        L.tm_set_class_tuning.argtypes = [P, I32, ctypes.POINTER(Tuning)]
         L.tm_batch_prepare.argtypes = [P, I32, P, P, P, P, ctypes.POINTER(Tuning), ctypes.POINTER(P)]
         L.tm_batch_run.argtypes = [P, P]
         L.tm_batch_free.argtypes = [P]
@@ -86,7 +88,7 @@ def lib():
         L.tm_server_destroy.argtypes = [P]
         for name in ("tm_graph_upload", "tm_graph_free", "tm_graph_info", "tm_count", "tm_enum",
                      "tm_batch_count", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work",
-                     "tm_batch_prepare", "tm_batch_run", "tm_batch_free",
+                     "tm_batch_prepare", "tm_batch_run", "tm_batch_free", "tm_shape_class", "tm_set_class_tuning",
                      "tm_server_create", "tm_server_submit", "tm_server_wait", "tm_server_take_rows",
                      "tm_server_release", "tm_server_stats", "tm_server_destroy"):
             getattr(L, name).restype = ctypes.c_int
@@ -118,6 +120,14 @@ def tuning(**kw):
 
 def tuning_dict(t):
     return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
+
+
+def shape_class(edges):
+    """Shape class of a motif (stack depth and node-map width of its launch)."""
+    src, dst = motif_arrays(edges)
+    c = ctypes.c_int32()
+    check(lib().tm_shape_class(len(edges), _p(src), _p(dst), ctypes.byref(c)))
+    return c.value
 
 
 def motif_arrays(edges):
@@ -156,7 +166,7 @@ class DeviceGraph:
         st = np.zeros(16, dtype=np.uint64)
         check(lib().tm_last_stats(self.handle, _p(st), 16))
         return dict(steps=int(st[0]), items=int(st[1]), roots=int(st[2]), leaves=int(st[3]), pushes=int(st[4]),
-                    donated=int(st[8]), taken=int(st[9]), capexit=int(st[10]), polls=int(st[11]),
+                    donated=int(st[8]), taken=int(st[9]), capexit=int(st[10]), work_ticks=int(st[11]),
                     grid=int(st[12]), claim_blocks=int(st[13]), lds=int(st[14]), wpe=int(st[15]))
 
     def last_kernel_ms(self):
@@ -201,6 +211,10 @@ class DeviceGraph:
         check(lib().tm_batch_count(self.handle, len(queries), _p(ne), _p(src), _p(dst), _p(dl), ctypes.byref(t),
                                    _p(out)))
         return [int(x) for x in out]
+
+    def set_class_tuning(self, cls, tune):
+        """Install (tune: a Tuning) or clear (None) the tuning of a shape class."""
+        check(lib().tm_set_class_tuning(self.handle, int(cls), ctypes.byref(tune) if tune is not None else None))
 
     def prepare_batch(self, queries, tune=None):
         """A PreparedBatch: the batch's launches captured into a HIP graph."""

@@ -316,11 +316,14 @@ static tm_status dispatch_n(Slot &sl, bool enumerate, bool instr, int wpe, bool 
                             const tm_kernel_args *a, int grid) {
   if (instr) return variant<OCC, MAXN, false, true, 1, false>(sl, lds, blocks, a, grid);
   if (enumerate) return variant<OCC, MAXN, true, false, 1, false>(sl, lds, blocks, a, grid);
-  if (multi)
-    return wpe == 6 ? variant<OCC, MAXN, false, false, 6, true>(sl, lds, blocks, a, grid)
-                    : variant<OCC, MAXN, false, false, 1, true>(sl, lds, blocks, a, grid);
-  return wpe == 6 ? variant<OCC, MAXN, false, false, 6, false>(sl, lds, blocks, a, grid)
-                  : variant<OCC, MAXN, false, false, 1, false>(sl, lds, blocks, a, grid);
+  if (multi) {
+    if (wpe == 6) return variant<OCC, MAXN, false, false, 6, true>(sl, lds, blocks, a, grid);
+    if (wpe == 5) return variant<OCC, MAXN, false, false, 5, true>(sl, lds, blocks, a, grid);
+    return variant<OCC, MAXN, false, false, 1, true>(sl, lds, blocks, a, grid);
+  }
+  if (wpe == 6) return variant<OCC, MAXN, false, false, 6, false>(sl, lds, blocks, a, grid);
+  if (wpe == 5) return variant<OCC, MAXN, false, false, 5, false>(sl, lds, blocks, a, grid);
+  return variant<OCC, MAXN, false, false, 1, false>(sl, lds, blocks, a, grid);
 }
 
 template <bool OCC>
@@ -350,21 +353,23 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
   const int nmap = maxN <= 4 ? 4 : 8;
-  const int per_frame = 6 + nmap + (enumerate ? TM_MAXE : 0);
-  // frame stack (an even number of ints), then the launch's per-query counts
-  const int lds = frames * per_frame * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
+  const int per_frame = 3 + nmap + (enumerate ? TM_MAXE : 0);  // cur, end, level word, node map, positions
+  // frame stack (an even number of ints), the roots' window table (2 x 65
+  // ints), then the launch's per-query counts
+  const int lds = (frames * per_frame + 2 * tmk::TM_ROOTS) * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
   int blocks = 0;
   tm_status st;
-  int wpe = (tune && tune->waves_per_simd_hint > 0) ? (tune->waves_per_simd_hint >= 6 ? 6 : 1) : 0;
+  int wpe = 0;  // register cap of the counting kernel: 1 (none), 5 (96 VGPRs) or 6 (80 VGPRs)
+  if (tune && tune->waves_per_simd_hint > 0)
+    wpe = tune->waves_per_simd_hint >= 6 ? 6 : (tune->waves_per_simd_hint == 5 ? 5 : 1);
   if (fn) {
     int32_t b = 0;
     if (fn(0, nullptr, 0, lds, sl.stream, &b, user) != 0) return fail(TM_ERR_HIP, "custom kernel occupancy query failed");
     blocks = b;
   } else {
-    if (wpe == 0) {  // auto: the register cap pays off only where LDS is not the limit
-      int b6 = 0;
-      if ((st = dispatch<true>(sl, nmap, enumerate, instr, 6, multi, lds, &b6, nullptr, 0)) != TM_OK) return st;
-      wpe = b6 >= 24 ? 6 : 1;
+    if (wpe == 0) {
+      wpe = 5;  // measured best on the 64M-edge set over 15 heavy queries (5: 7.00 s, 6: 7.81 s, 1: 7.68 s);
+                // the autotuner picks 6 per shape where it wins
     }
     if ((st = dispatch<true>(sl, nmap, enumerate, instr, wpe, multi, lds, &blocks, nullptr, 0)) != TM_OK) return st;
   }
@@ -859,9 +864,33 @@ static tm_status count_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc,
   return finish(g, sl, 0, 1, count);
 }
 
+static int shape_class(int M, int n);
+static const tm_tuning *class_tuning(tm_graph *g, int cls, const tm_tuning *tune);
+static int motif_class(int32_t num_edges, const int32_t *msrc, const int32_t *mdst);
+
 tm_status tm_count(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
                    int32_t beg, int32_t end, const tm_tuning *tune, uint64_t *count) {
+  if (g && msrc && mdst && num_edges >= 2 && num_edges <= TM_MAX_MOTIF_EDGES)
+    tune = class_tuning(g, motif_class(num_edges, msrc, mdst), tune);
   return count_impl(g, num_edges, msrc, mdst, delta, beg, end, tune, count, nullptr, nullptr);
+}
+
+tm_status tm_shape_class(int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t *cls) {
+  if (!msrc || !mdst || !cls || num_edges < 2 || num_edges > TM_MAX_MOTIF_EDGES) return fail(TM_ERR_ARG, "bad argument");
+  *cls = motif_class(num_edges, msrc, mdst);
+  return TM_OK;
+}
+
+tm_status tm_set_class_tuning(tm_graph *g, int32_t cls, const tm_tuning *tune) {
+  if (!g || cls < 0 || cls >= TM_NCLASSES) return fail(TM_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(g->mtx);
+  if (tune) {
+    g->class_tune[cls] = *tune;
+    g->class_set[cls] = true;
+  } else {
+    g->class_set[cls] = false;
+  }
+  return TM_OK;
 }
 
 tm_status tm_count_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
@@ -892,6 +921,20 @@ tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, con
 // Launch shape class of a count query: frame-stack levels and node-map width
 // (they set the LDS a wavefront needs, hence the occupancy of its launch).
 static int shape_class(int M, int n) { return std::max(1, M - 2) * 2 + (n <= 4 ? 0 : 1); }
+
+// The graph's tuning table: a call without launch parameters of its own uses
+// the entry of its query's shape class, if one was set.
+static const tm_tuning *class_tuning(tm_graph *g, int cls, const tm_tuning *tune) {
+  if (tune) return tune;
+  if (cls >= 0 && cls < TM_NCLASSES && g->class_set[cls]) return &g->class_tune[cls];
+  return nullptr;
+}
+
+static int motif_class(int32_t num_edges, const int32_t *msrc, const int32_t *mdst) {
+  int n = 0;
+  for (int k = 0; k < num_edges; k++) n = std::max(n, 1 + std::max(msrc[k], mdst[k]));
+  return shape_class(num_edges, n);
+}
 
 tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, const int32_t *msrc,
                          const int32_t *mdst, const int32_t *deltas, const tm_tuning *tune, uint64_t *counts) {
@@ -930,7 +973,7 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
       Slot &sl = g->slot[k];
       st = stage_queries(g, sl, 0, gr.n, order.data() + gr.first, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas,
                          nullptr, nullptr);
-      if (st == TM_OK) st = launch(g, sl, 0, gr.n, false, tune, false, true);
+      if (st == TM_OK) st = launch(g, sl, 0, gr.n, false, class_tuning(g, key[order[gr.first]], tune), false, true);
       if (st == TM_OK) launched++;
     }
     for (size_t k = 0; k < launched; k++) {  // wait for every launched group, even after an error

@@ -62,6 +62,8 @@ struct tm_graph {
   tm_graph_view view{};
   tmh::Slot slot[TM_SLOTS];  // slot 0 also carries the upload
   int last_slot = 0;         // slot of the last finished launch (stats, kernel time)
+  tm_tuning class_tune[TM_NCLASSES] = {};  // per shape class (autotuner), used when a call brings none
+  bool class_set[TM_NCLASSES] = {};
   std::mutex mtx;            // one API call at a time per graph handle
 };
 

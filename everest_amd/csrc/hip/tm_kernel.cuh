@@ -28,6 +28,7 @@ namespace tmk {
 
 constexpr int WAVE = 64;
 constexpr int TM_BLOCK = 4 * WAVE;  // roots per dispatch block (4 per lane)
+constexpr int TM_ROOTS = WAVE + 1;  // window table entries: one per lane's root, one for a taken frame
 
 // ------------------------------------------------------------------ helpers
 
@@ -358,16 +359,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   extern __shared__ int lds[];
   const int lane = threadIdx.x;
   const int FR = args.frames;
+  // frame f: candidate slots [f_cur, f_end) feeding the next motif edge,
+  // level | direction << 8 | root << 16, node map, edge positions (enumeration).
+  // The window of a frame (pend, tl) is its root's, kept once per root in
+  // r_pend / r_tl: entries 0..63 for the roots of the quarter being
+  // expanded (lane order), entry 64 for a frame taken from another wavefront.
   int *f_cur = lds;
   int *f_end = f_cur + FR;
-  int *f_p = f_end + FR;
-  int *f_pend = f_p + FR;
-  int *f_lvl = f_pend + FR;
-  int *f_tl = f_lvl + FR;
-  int *f_m = f_tl + FR;
+  int *f_lvl = f_end + FR;
+  int *f_m = f_lvl + FR;
   int *f_e = f_m + MAXN * FR;
+  int *r_pend = f_e + (ENUM ? MAXE * FR : 0);
+  int *r_tl = r_pend + TM_ROOTS;
   // per-query counts of this wavefront, flushed once at the end
-  unsigned long long *q_cnt = (unsigned long long *)(f_e + (ENUM ? MAXE * FR : 0));
+  unsigned long long *q_cnt = (unsigned long long *)(r_tl + TM_ROOTS);
   for (int q = lane; q < args.nq; q += WAVE) q_cnt[q] = 0;
   __syncthreads();
 
@@ -380,7 +385,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   unsigned long long st_steps = 0, st_items = 0, st_roots = 0, st_push = 0;  // uniform
   unsigned st_leaves = 0;  // per lane
   unsigned long long rx1 = 0, rx2 = 0, rp = 0;  // reference work (INSTR only)
-  unsigned long long st_don = 0, st_take = 0, st_capexit = 0, st_polls = 0;
+  unsigned long long st_don = 0, st_take = 0, st_capexit = 0;
+  unsigned long long st_work = 0;            // wall-clock ticks this wavefront held work
+  long long work_t0 = (long long)wall_clock64();
   unsigned iter = 0;
   int qcur = -1;
   int top = 0;
@@ -403,9 +410,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     if (f == 1) return &f_lvl[w];
     if (f == 2) return &f_cur[w];
     if (f == 3) return &f_end[w];
-    if (f == 4) return &f_p[w];
-    if (f == 5) return &f_pend[w];
-    if (f == 6) return &f_tl[w];
+    if (f == 5) return &r_pend[(f_lvl[w] >> 16) & 0xff];
+    if (f == 6) return &r_tl[(f_lvl[w] >> 16) & 0xff];
     if (f >= 7 && f < 7 + MAXN) return &f_m[(f - 7) * FR + w];
     if (ENUM && f >= 7 + TM_MAXN && f < 7 + TM_MAXN + MAXE) return &f_e[(f - 7 - TM_MAXN) * FR + w];
     return nullptr;
@@ -432,6 +438,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       if (!is_idle) {
         if (lane == 0) add_rlx(&sched->busy, -1);
         is_idle = true;
+        st_work += (unsigned long long)((long long)wall_clock64() - work_t0);
       }
       if (!waiter) {
         // at most idle_cap wavefronts of a launch wait for donations (one
@@ -458,7 +465,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       int fv = 0;
       bool got = false;
       for (unsigned spin = 0;; spin++) {
-        st_polls++;
         unsigned long long word = lane < TM_POOL_FIELDS ? ld_rlx(rec + lane) : 0ull;
         const bool mine = lane >= TM_POOL_FIELDS || (unsigned)(word >> 32) == tag;
         fv = (int)(unsigned)(word & 0xffffffffull);
@@ -488,6 +494,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       waiter = false;
       st_take++;
       is_idle = false;  // the record's busy credit is now this wavefront's
+      work_t0 = (long long)wall_clock64();
       const int fq = uni(__shfl(fv, 0));
       if (fq != qcur) {
         if (!ENUM && qcur >= 0) {
@@ -497,7 +504,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         cnt = 0;
         qcur = fq;
       }
-      if (lane < TM_POOL_FIELDS) {
+      // the taken frame becomes frame 0, its window root entry TM_ROOTS - 1
+      if (lane == 1) f_lvl[0] = (fv & 0xffff) | ((TM_ROOTS - 1) << 16);
+      __syncthreads();
+      if (lane < TM_POOL_FIELDS && lane != 1) {
         int *dst = frame_field(lane, 0);
         if (dst) *dst = fv;
       }
@@ -638,8 +648,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       int n = __popcll(bm);
       if (top + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
       if (ok) {
+        r_pend[lane] = pend;  // the window of this lane's root (no frame of an older root is left)
+        r_tl[lane] = tli;
         int w = top + __popcll(bm & lt_mask);
-        f_cur[w] = cb; f_end[w] = ce; f_p[w] = r; f_pend[w] = pend; f_lvl[w] = 0 | (dir << 8); f_tl[w] = tli;
+        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = 0 | (dir << 8) | (lane << 16);
 #pragma unroll
         for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
         if constexpr (ENUM) f_e[w] = r;
@@ -689,17 +701,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const int myf = top - 1 - rank;
     st_steps++;
     st_items += total;
-    int slot = 0, p = 0, pend = 0, dir = 0, tl = 0;
+    int slot = 0, pend = 0, dir = 0, tl = 0, ridx = 0;
     int m[MAXN];
     int e[ENUM ? MAXE : 1];
 #pragma unroll
     for (int i = 0; i < MAXN; i++) m[i] = 0;
     if (has) {
       slot = f_cur[myf] + off;
-      p = f_p[myf];
-      pend = f_pend[myf];
-      tl = f_tl[myf];
-      dir = f_lvl[myf] >> 8;
+      const int lv = f_lvl[myf];
+      dir = (lv >> 8) & 0xff;
+      ridx = (lv >> 16) & 0xff;
+      pend = r_pend[ridx];
+      tl = r_tl[ridx];
 #pragma unroll
       for (int i = 0; i < MAXN; i++) m[i] = f_m[i * FR + myf];
       if constexpr (ENUM) {
@@ -782,7 +795,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     if (ntop + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
     if (valid) {
       int w = ntop + __popcll(bm & lt_mask);
-      f_cur[w] = cb; f_end[w] = ce; f_p[w] = pos; f_pend[w] = pend; f_lvl[w] = j | (cdir << 8); f_tl[w] = tl;
+      f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | (ridx << 16);
 #pragma unroll
       for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
       if constexpr (ENUM) {
@@ -864,6 +877,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   if (!ENUM)
     for (int q = lane; q < args.nq; q += WAVE)
       if (q_cnt[q]) atomicAdd(args.queries[q].count, q_cnt[q]);
+  if (!is_idle) st_work += (unsigned long long)((long long)wall_clock64() - work_t0);
   if (lane == 0 && !is_idle) add_rlx(&sched->busy, -1);  // left early holding work (error or deadline)
   if (lane == 0 && waiter) add_rlx(&sched->waiters, -1);
   // work counters: plain stores into this wavefront's own slot, summed by a
@@ -878,7 +892,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       case TM_STAT_DONATED: v = st_don; break;
       case TM_STAT_TAKEN: v = st_take; break;
       case TM_STAT_CAPEXIT: v = st_capexit; break;
-      case TM_STAT_POLLS: v = st_polls; break;
+      case TM_STAT_WORK: v = st_work; break;
       default: break;
     }
     const unsigned long long leaves = wave_sum64(st_leaves);

@@ -39,7 +39,7 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 
 // Depth class of a count request: launches are formed within one class, so
 // no launch sizes its LDS stack for a deeper motif than its own.
-int depth_class(int M, int n) { return std::max(1, M - 2) * 2 + (n <= 4 ? 0 : 1); }
+int depth_class(int M, int n) { return std::max(1, M - 2) * 2 + (n <= 4 ? 0 : 1); }  // as tm_shape_class
 
 struct Request {
   int M = 0;
@@ -114,6 +114,7 @@ struct tm_server {
   int slots = TM_SLOTS, max_batch = 256;
   size_t arena_bytes = 0;
   tm_tuning tune{};
+  bool own_tune = false;  // else the graph's per-class table
   std::mutex mu;
   std::condition_variable cv_done, cv_work;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
@@ -189,7 +190,12 @@ bool launch_next(tm_server *s, int k) {
     sl.s.h_queries[0].rows = (int *)(s->arena.base + r.arena_off);
     sl.s.h_queries[0].cap = r.cap;
   }
-  if (st == TM_OK) st = launch(s->g, sl, 0, n, enumerate, &s->tune, false, !enumerate);
+  // launch parameters: the server's own, else the graph's entry for the class
+  const int cls = s->reqs[pick[0]]->cls;
+  const tm_tuning *tune = s->own_tune ? &s->tune
+                                      : ((cls >= 0 && cls < TM_NCLASSES && s->g->class_set[cls]) ? &s->g->class_tune[cls]
+                                                                                                : nullptr);
+  if (st == TM_OK) st = launch(s->g, sl, 0, n, enumerate, tune, false, !enumerate);
   hipError_t e = hipSuccess;
   if (st == TM_OK) e = hipMemcpyAsync(ln.h_sched, sl.s.d_sched, sizeof(tm_sched), hipMemcpyDeviceToHost, sl.stream);
   if (st == TM_OK && e == hipSuccess)
@@ -311,6 +317,8 @@ tm_status tm_server_create(tm_graph *g, const tm_server_config *cfg, tm_server *
     if (cfg->max_batch) s->max_batch = cfg->max_batch;
     s->arena_bytes = cfg->arena_bytes ? (size_t)cfg->arena_bytes : (size_t)1 << 30;
     s->tune = cfg->tune;
+    const tm_tuning zero{};
+    s->own_tune = memcmp(&cfg->tune, &zero, sizeof(zero)) != 0;
   } else {
     s->arena_bytes = (size_t)1 << 30;
   }

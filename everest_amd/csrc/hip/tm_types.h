@@ -126,7 +126,7 @@ typedef struct {
 } tm_sched;
 
 // A frame donated by a busy wavefront to an idle one: fields q, lvl, cur,
-// end, p, pend, tl, m[TM_MAXN], e[TM_MAXE], each stored as a 64-bit word
+// end, (unused), pend, tl, m[TM_MAXN], e[TM_MAXE], each stored as a 64-bit word
 // (tag << 32 | value) with device-scope atomic stores, so a taker knows the
 // record is complete when every word carries the tag it expects: no fences,
 // no cache invalidation.  Records live in a ring of pool_cap slots; record r
@@ -148,7 +148,7 @@ enum {
   TM_STAT_DONATED = 8, // frames donated to the pool
   TM_STAT_TAKEN = 9,   // donated frames taken
   TM_STAT_CAPEXIT = 10,// wavefronts that left because enough were waiting
-  TM_STAT_POLLS = 11,  // pool polls by waiting wavefronts
+  TM_STAT_WORK = 11,   // wall-clock ticks wavefronts held work (summed)
   TM_STAT_GRID = 12,   // launch shape, filled in by the host: wavefronts,
   TM_STAT_CLAIM = 13,  //   256-root blocks per claim,
   TM_STAT_LDS = 14,    //   LDS bytes per wavefront,

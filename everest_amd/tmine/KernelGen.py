@@ -104,10 +104,11 @@ class KernelGen:
             t.update({k: int(v) for k, v in tuning.items() if k in t})
         # register cap of the counting kernel: 80 VGPRs (6 wavefronts per SIMD)
         # when the LDS stack of this motif leaves room for them (at most two
-        # stacked levels), else the compiler's allocation
+        # stacked levels), else 96 VGPRs (5 per SIMD)
         if t["waves_per_simd_hint"] <= 0:
-            t["waves_per_simd_hint"] = 6 if len(self.edges) - 2 <= 2 else 1
-        t["wpe"] = 6 if t["waves_per_simd_hint"] >= 6 else 1
+            t["waves_per_simd_hint"] = 6 if len(self.edges) - 2 <= 2 else 5
+        h = t["waves_per_simd_hint"]
+        t["wpe"] = 6 if h >= 6 else (5 if h == 5 else 1)
         t["waves_per_simd_hint"] = t["wpe"]
         if not self.options.opts.get("warpBalancing", True):
             t["steal_every"] = 1 << 30  # no work donation between wavefronts

@@ -72,7 +72,9 @@ typedef struct tm_tuning {
   int32_t steal_min;     /* a donated half-frame holds at least this many candidates */
   int32_t steal_after_us; /* a wavefront donates only after holding one task this long */
   int32_t waves_per_simd_hint; /* counting kernel variant: 1 = compiler's register allocation,
-                                  6 = capped at 80 VGPRs (24 wavefronts per CU); 0 = pick by LDS */
+                                  5 = capped at 96 VGPRs (20 wavefronts per CU), 6 = capped at 80
+                                  VGPRs (24 per CU); 0 = 6 where the LDS stack allows 4 more
+                                  wavefronts per CU than 5, else 5 */
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */
@@ -99,7 +101,8 @@ tm_status tm_graph_free(tm_graph *g);
  * items examined, [2] roots dispatched, [3] last-edge range computations,
  * [4] frames pushed, [5..7] reference work of an instrumented run (X1, X2, P),
  * [8] frames donated between wavefronts, [9] donated frames taken, [10]
- * wavefronts that left early, [11] pool polls, and the launch shape: [12]
+ * wavefronts that left early, [11] wall-clock ticks (100 MHz) wavefronts held
+ * work, summed, and the launch shape: [12]
  * wavefronts in the grid, [13] blocks per claim, [14] LDS bytes per
  * wavefront, [15] register-cap variant.  Up to 16 entries.
  * No reference counterpart (instrumentation).
@@ -178,6 +181,18 @@ tm_status tm_enum_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, co
                          int32_t delta, int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune,
                          tm_launch_fn launcher, void *user, int32_t *rows, uint64_t *count,
                          int64_t *rows_written);
+
+/*
+ * Shape classes and the graph's tuning table (the autotuner's output).  A
+ * motif's class is 2 * max(1, M - 2) + (nodes > 4): the LDS stack depth and
+ * node-map width of its launch, which set its occupancy.  tm_count,
+ * tm_batch_count and the server use the class's entry when they are called
+ * without launch parameters (tune == NULL).  tune == NULL in
+ * tm_set_class_tuning clears the entry.  No reference counterpart.
+ */
+#define TM_NCLASSES 16
+tm_status tm_shape_class(int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t *cls);
+tm_status tm_set_class_tuning(tm_graph *g, int32_t cls, const tm_tuning *tune);
 
 /*
  * Count many motif queries on one graph in one call.  Query q uses edges
