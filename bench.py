@@ -171,13 +171,15 @@ def main():
         kname, dom_n, dom_bytes / 1e9, dom_ms, achieved))
     # HBM bytes per launch of that kernel on this workload, from the committed
     # counter summary (two rocprofv3 --pmc runs, scaled by a 1 GiB copy)
+    # (per query, so the mean covers exactly this run's launches of that kernel)
     traffic, traffic_src = None, None
     tf = os.path.join(ROOT, "profiles", "mine_kernel_pl64m_config4_hbm.json")
     if os.path.exists(tf) and args.graph == "pl64m" and args.queries == 256 and args.seed == 4:
         with open(tf) as f:
             t = json.load(f)
-        if t.get("kernel") == kname and t.get("launches") == dom_n:
-            traffic = round(t["hbm_bytes_per_launch"], 1)
+        perq = t.get("hbm_bytes_per_query", {})
+        if t.get("kernel") == kname and all(str(i) in perq for i in groups[dom]):
+            traffic = round(sum(perq[str(i)] for i in groups[dom]) / dom_n, 1)
             traffic_src = "profiles/mine_kernel_pl64m_config4_hbm.md (FETCH_SIZE x %.3f + WRITE_SIZE x %.3f)" % (
                 t["fetch_scale"], t["write_scale"])
 
