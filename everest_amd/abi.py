@@ -37,7 +37,8 @@ class Tuning(ctypes.Structure):
                 ("scan_max", ctypes.c_int32), ("time_limit_ms", ctypes.c_int32),
                 ("steal_every", ctypes.c_int32), ("steal_max", ctypes.c_int32),
                 ("steal_min", ctypes.c_int32), ("steal_after_us", ctypes.c_int32),
-                ("waves_per_simd_hint", ctypes.c_int32)]
+                ("waves_per_simd_hint", ctypes.c_int32), ("sb_flush", ctypes.c_int32),
+                ("sb_inline", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):

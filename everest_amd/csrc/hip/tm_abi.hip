@@ -354,24 +354,55 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   const int frames = 64 * lvls;
   const int nmap = maxN <= 4 ? 4 : 8;
   const int per_frame = 3 + nmap + (enumerate ? TM_MAXE : 0);  // cur, end, level word, node map, positions
+  // survivor compaction of the counting kernel: survivors buffered before a
+  // full-width expansion (0: expanded in the step that filtered them), and
+  // the survivors with which a step expands its own when nothing is buffered
+  int sb_flush = 0, sb_inline = 0;
+  bool sb_auto = false;  // the library's choice: on unless its buffer costs resident wavefronts
+  if (!enumerate && !instr) {
+    sb_flush = TM_SB_FLUSH_DEFAULT;
+    sb_inline = TM_SB_INLINE_DEFAULT;
+    sb_auto = !(tune && tune->sb_flush != 0);
+    if (!sb_auto) sb_flush = tune->sb_flush < 0 ? 0 : std::min(tune->sb_flush, tmk::WAVE);
+    if (tune && tune->sb_inline > 0) sb_inline = std::min(tune->sb_inline, tmk::WAVE);
+  }
   // frame stack (an even number of ints), the roots' window table (2 x 65
-  // ints), then the launch's per-query counts
-  const int lds = (frames * per_frame + 2 * tmk::TM_ROOTS) * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
-  int blocks = 0;
-  tm_status st;
+  // ints), the survivor buffer (counting), then the launch's per-query counts
+  auto lds_of = [&](int flush) {
+    const int sb = flush > 0 ? (2 + nmap) * tmk::WAVE : 0;
+    return (frames * per_frame + 2 * tmk::TM_ROOTS + sb) * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
+  };
   int wpe = 0;  // register cap of the counting kernel: 1 (none), 5 (96 VGPRs) or 6 (80 VGPRs)
   if (tune && tune->waves_per_simd_hint > 0)
     wpe = tune->waves_per_simd_hint >= 6 ? 6 : (tune->waves_per_simd_hint == 5 ? 5 : 1);
-  if (fn) {
-    int32_t b = 0;
-    if (fn(0, nullptr, 0, lds, sl.stream, &b, user) != 0) return fail(TM_ERR_HIP, "custom kernel occupancy query failed");
-    blocks = b;
-  } else {
-    if (wpe == 0) {
-      wpe = 5;  // measured best on the 64M-edge set over 15 heavy queries (5: 7.00 s, 6: 7.81 s, 1: 7.68 s);
-                // the autotuner picks 6 per shape where it wins
+  if (!fn && wpe == 0) {
+    wpe = 5;  // measured best on the 64M-edge set over 15 heavy queries (5: 7.00 s, 6: 7.81 s, 1: 7.68 s);
+              // the autotuner picks 6 per shape where it wins
+  }
+  auto blocks_of = [&](int lds_bytes, int *blocks) -> tm_status {
+    if (fn) {
+      int32_t b = 0;
+      if (fn(0, nullptr, 0, lds_bytes, sl.stream, &b, user) != 0)
+        return fail(TM_ERR_HIP, "custom kernel occupancy query failed");
+      *blocks = b;
+      return TM_OK;
     }
-    if ((st = dispatch<true>(sl, nmap, enumerate, instr, wpe, multi, lds, &blocks, nullptr, 0)) != TM_OK) return st;
+    return dispatch<true>(sl, nmap, enumerate, instr, wpe, multi, lds_bytes, blocks, nullptr, 0);
+  };
+  int lds = lds_of(sb_flush);
+  int blocks = 0;
+  tm_status st;
+  if ((st = blocks_of(lds, &blocks)) != TM_OK) return st;
+  if (sb_auto && sb_flush > 0) {
+    int b0 = 0;
+    if ((st = blocks_of(lds_of(0), &b0)) != TM_OK) return st;
+    const int cap = (tune && tune->waves_per_cu > 0) ? tune->waves_per_cu : 32;
+    if (std::min(b0, cap) > std::min(blocks, cap)) {
+      // the buffer would cost resident wavefronts: measured to cost more than it saves
+      sb_flush = 0;
+      lds = lds_of(0);
+      blocks = b0;
+    }
   }
   int wpc = (tune && tune->waves_per_cu > 0) ? tune->waves_per_cu : 32;
   if (blocks > 0) wpc = std::min(wpc, blocks);
@@ -422,6 +453,8 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   a.total_blocks = total_blocks;
   a.claim_blocks = claim_blocks;
   a.frames = frames;
+  a.sb_flush = sb_flush;
+  a.sb_inline = sb_inline;
   a.scan_max = (tune && tune->scan_max > 0) ? tune->scan_max : 16;
   a.limit_ticks = 0;
   if (tune && tune->time_limit_ms > 0) a.limit_ticks = (long long)tune->time_limit_ms * (g->wall_rate_khz);

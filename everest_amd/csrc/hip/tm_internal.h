@@ -10,6 +10,8 @@
 #include "tm_types.h"
 
 #define TM_LAUNCH_MAXQ 512  // queries per launch (their table and counts live in LDS)
+#define TM_SB_FLUSH_DEFAULT 32   // counting: survivors buffered before a full-width expansion
+#define TM_SB_INLINE_DEFAULT 32  // counting: survivors a step expands itself when nothing is buffered
 
 namespace tmh {
 

@@ -371,8 +371,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   int *f_e = f_m + MAXN * FR;
   int *r_pend = f_e + (ENUM ? MAXE * FR : 0);
   int *r_tl = r_pend + TM_ROOTS;
+  // survivor buffer (counting, when args.sb_flush > 0): candidates of one
+  // level that passed the injectivity and pruning tests, waiting to be
+  // expanded up to 64 at a time
+  const bool COMPACT = !ENUM && !INSTR && args.sb_flush > 0;
+  int *sb_pos = r_tl + TM_ROOTS;
+  int *sb_ridx = sb_pos + (COMPACT ? WAVE : 0);
+  int *sb_m = sb_ridx + (COMPACT ? WAVE : 0);
   // per-query counts of this wavefront, flushed once at the end
-  unsigned long long *q_cnt = (unsigned long long *)(r_tl + TM_ROOTS);
+  unsigned long long *q_cnt = (unsigned long long *)(sb_m + (COMPACT ? MAXN * WAVE : 0));
   for (int q = lane; q < args.nq; q += WAVE) q_cnt[q] = 0;
   __syncthreads();
 
@@ -404,6 +411,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   long long task_t0 = 0;   // wall clock when the current task (root group or donated frame) began
   bool is_idle = false;    // gave up its busy credit (sched->busy starts at the grid size)
   bool waiter = false;     // counted in sched->waiters
+  int sbn = 0;             // survivors buffered (uniform)
+  int sb_level = 0;        // level of the frames they came from
 
   // copy one frame between the LDS stack (slot w) and a pool record, one field per lane
   auto frame_field = [&](int f, int w) -> int * {
@@ -426,7 +435,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         break;
       }
     }
-    if (top == 0 && roots_done) {
+    if (top == 0 && sbn == 0 && roots_done) {
       // ------------------------------------------ idle: take donated work
       // Ticket protocol: an idle wavefront draws ticket t (one atomic on
       // pool_head) and waits for record t.  Donors fill records in order
@@ -516,7 +525,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       __syncthreads();
       continue;
     }
-    if (top == 0) {
+    if (top == 0 && sbn == 0) {
       // ---------------------------------------------------- root dispatch
       // The launch's roots form one space of 256-root blocks, query after
       // query, handed out in fixed claims over striped counters.
@@ -666,7 +675,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const tm_query_dev &q = args.queries[qcur];
     const tm_motif_dev &md = q.m;
     const int M = SP::M(md);
-    const int L = uni(f_lvl[top - 1] & 0xff);
+    const int last = ENUM ? M - 1 : M - 2;  // last motif edge handled per candidate
+    const int Ltop = top > 0 ? uni(f_lvl[top - 1] & 0xff) : -1;
+    bool flush = COMPACT && sbn > 0 && (top == 0 || Ltop != sb_level || sbn >= args.sb_flush);
+    if (!flush) {
+    // ---- deal the next candidate slots of the topmost frames of one level
+    // to the lanes (at most lim: room left in the survivor buffer)
+    const int L = Ltop;
+    const int lim = COMPACT ? WAVE - sbn : WAVE;
     const int fi = top - 1 - lane;
     int fcnt = 0;
     if (fi >= 0) {
@@ -676,19 +692,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     int incl = wave_incl_scan(fcnt, lane);
     int excl = incl - fcnt;
     int total = uni(__shfl(incl, WAVE - 1));
-    if (total > WAVE) total = WAVE;
+    if (total > lim) total = lim;
     if (total <= 0) {  // a frame on the stack is never empty; stop rather than spin
       if (lane == 0) atomicOr(&sched->error, 2);
       break;
     }
-    const bool part = (fcnt > 0) && (excl < WAVE);
+    const bool part = (fcnt > 0) && (excl < lim);
     const unsigned long long pm = __ballot(part);
     const int nparts = __popcll(pm);
-    int take = part ? min(fcnt, WAVE - excl) : 0;
+    int take = part ? min(fcnt, lim - excl) : 0;
     const bool last_partial = __ballot(part && take < fcnt) != 0ull;
 
     int rank, off;
-    if (uni(__shfl(fcnt, 0)) >= WAVE) {
+    if (uni(__shfl(fcnt, 0)) >= lim) {
       rank = 0; off = lane;
     } else {
       unsigned long long sm = wave_or64(part ? (1ull << excl) : 0ull);
@@ -750,7 +766,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         valid = true;
       }
     }
-    const int last = ENUM ? M - 1 : M - 2;
+    // survivors are buffered while they are few and more of this level
+    // follows; otherwise (nothing buffered, and sb_inline survivors or the
+    // level's last candidates in hand) they are expanded in this step
+    bool buffer = false;
+    if (COMPACT) {
+      const bool surv = valid && (tmn < 0 || tfn <= tl);
+      const unsigned long long bm = __ballot(surv);
+      const int ns = __popcll(bm);
+      const bool level_done = ntop == 0 || uni(f_lvl[ntop - 1] & 0xff) != L;
+      buffer = sbn > 0 || (ns < args.sb_inline && !level_done);
+      if (buffer) {
+        if (surv) {
+          const int w = sbn + __popcll(bm & lt_mask);
+          sb_pos[w] = pos;
+          sb_ridx[w] = ridx;
+#pragma unroll
+          for (int i = 0; i < MAXN; i++) sb_m[i * WAVE + w] = m[i];
+        }
+        sbn += ns;
+        sb_level = L;
+        top = ntop;
+        __syncthreads();
+        flush = sbn > 0 && (level_done || sbn >= args.sb_flush);
+      }
+    }
+    if (!buffer) {
     if constexpr (INSTR) {
       if (valid && j + 1 < M) {
         (SP::kind(md, j + 1) == TM_K_CLOSE ? rx2 : rx1) += 1;
@@ -808,6 +849,58 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     st_push += n;
     top = ntop + n;
     __syncthreads();
+    }  // expanded in this step
+    }  // deal and filter
+    if (flush) {
+      // ---- expand the buffered survivors, one per lane: the long chains of
+      // dependent loads (candidate list of the next edge, window end, leaf
+      // counts) run with every lane busy instead of once per filter step
+      const int j = sb_level + 1;  // motif edge of the buffered candidates
+      const bool has = lane < sbn;
+      int pos = 0, ridx = 0, pend = 0;
+      int m[MAXN];
+#pragma unroll
+      for (int i = 0; i < MAXN; i++) m[i] = 0;
+      if (has) {
+        pos = sb_pos[lane];
+        ridx = sb_ridx[lane];
+        pend = r_pend[ridx];
+#pragma unroll
+        for (int i = 0; i < MAXN; i++) m[i] = sb_m[i * WAVE + lane];
+      }
+      st_steps++;
+      sbn = 0;
+      if (j == last) {
+        if (has) {
+          const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
+          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max);
+          st_leaves++;
+        }
+        __syncthreads();
+        continue;
+      }
+      int cb = 0, ce = 0, cdir = 0;
+      bool valid = has;
+      if (valid) {
+        const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
+        cb = LL.lo;
+        cdir = LL.dir;
+        ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend);
+        valid = ce > cb;
+      }
+      const unsigned long long bm = __ballot(valid);
+      const int n = __popcll(bm);
+      if (top + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
+      if (valid) {
+        const int w = top + __popcll(bm & lt_mask);
+        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | (ridx << 16);
+#pragma unroll
+        for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
+      }
+      st_push += n;
+      top += n;
+      __syncthreads();
+    }  // flush
     // ---------------------------------------------- donate to idle wavefronts
     // only a wavefront that has held its current task for steal_after_ticks
     // donates: short tasks never pay for the exchange

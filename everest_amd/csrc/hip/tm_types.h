@@ -170,6 +170,8 @@ typedef struct {
   int claim_blocks;           // blocks per claim (the same for every query of the launch)
   int frames;                 // LDS frame capacity per wavefront
   int scan_max;               // last-edge ranges up to this size are scanned
+  int sb_flush;               // counting: survivors buffered before a full-width expansion (0: none buffered)
+  int sb_inline;              // counting: with nothing buffered, a step with this many survivors expands them itself
   long long limit_ticks;      // wall-clock budget of the launch (0 = none)
   tm_sched *sched;
   unsigned long long *wstats; // TM_NSTATS counters per wavefront (workgroup)

@@ -99,7 +99,7 @@ class KernelGen:
         self.enumerate = self.options.enumerate()
         self.enum_NUM = self.options.enumNum() if self.enumerate else -1
         t = dict(waves_per_cu=0, claim_blocks=0, scan_max=0, steal_every=0, steal_max=0, steal_min=0,
-                 steal_after_us=0, waves_per_simd_hint=0)
+                 steal_after_us=0, waves_per_simd_hint=0, sb_flush=0, sb_inline=0)
         if tuning:
             t.update({k: int(v) for k, v in tuning.items() if k in t})
         # register cap of the counting kernel: 80 VGPRs (6 wavefronts per SIMD)

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define TM_ABI_VERSION 2
+#define TM_ABI_VERSION 3
 #define TM_MAX_MOTIF_EDGES 8
 #define TM_MAX_MOTIF_NODES 8
 
@@ -73,8 +73,12 @@ typedef struct tm_tuning {
   int32_t steal_after_us; /* a wavefront donates only after holding one task this long */
   int32_t waves_per_simd_hint; /* counting kernel variant: 1 = compiler's register allocation,
                                   5 = capped at 96 VGPRs (20 wavefronts per CU), 6 = capped at 80
-                                  VGPRs (24 per CU); 0 = 6 where the LDS stack allows 4 more
-                                  wavefronts per CU than 5, else 5 */
+                                  VGPRs (24 per CU); 0 = 5 */
+  int32_t sb_flush;      /* counting: candidates that pass the filters of one level are buffered in
+                            LDS and expanded together once this many (1..64) are held; -1 = each
+                            step expands its own survivors; 0 = the library's choice */
+  int32_t sb_inline;     /* counting: with nothing buffered, a step with at least this many survivors
+                            (1..64) expands them itself; 0 = the library's choice */
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */

@@ -111,6 +111,7 @@ def build_view(g):
 
 S_SEARCH, S_OUT_SELF, S_OUT_NEXT, S_IN_SELF, S_IN_NEXT, S_PAIR_SELF = range(6)
 TM_BLOCK = 4 * WAVE
+SB_FLUSH = 32  # survivors that trigger a full-width expansion (counting kernel; 0: none buffered)
 
 
 L_OUT, L_IN, L_OUTPAIR, L_INPAIR = range(4)
@@ -397,7 +398,8 @@ def ref_probes(g, md, k, m, p, pend):
     return (e - lo) + (1 if e < hi else 0)
 
 
-def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=256, cap=1 << 30, instr=False):
+def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=256, cap=1 << 30, instr=False,
+             sb_flush=SB_FLUSH, sb_inline=SB_FLUSH):
     """Simulate one wavefront mining roots [beg, end) alone; returns
     (count, rows, stats).  With instr, stats["ref"] = [R, X1, X2, P]."""
     M = md["M"]
@@ -452,10 +454,34 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
             for f in pushes:
                 frames[top] = f
                 top += 1
-            # ---- expansion steps until the stack drains
-            while top > 0:
+            # ---- expansion steps until the stack and the survivor buffer drain
+            compact = not enumerate_ and not instr and sb_flush > 0  # the counting kernel buffers survivors
+            sb, sb_level = [], 0
+            while top > 0 or sb:
                 stats["max_top"] = max(stats["max_top"], top)
-                L = frames[top - 1]["lvl"]
+                Ltop = frames[top - 1]["lvl"] if top > 0 else -1
+                if compact and sb and (top == 0 or Ltop != sb_level or len(sb) >= sb_flush):
+                    # full-width expansion of the buffered survivors
+                    assert len(sb) <= WAVE
+                    j = sb_level + 1
+                    children = []
+                    for pos, m, f in sb:
+                        Lc = cand_start(g, md, j + 1, m, pos)
+                        if j == last:
+                            cnt += leaf_count(g, md, j + 1, m, Lc, pos, f["pend"], scan_max)
+                            continue
+                        ce = gallop_lb(Lc[0], Lc[2], Lc[3], f["pend"])
+                        if ce > Lc[2]:
+                            children.append(dict(cur=Lc[2], end=ce, p=pos, pend=f["pend"], lvl=j, dir=Lc[4], m=m,
+                                                 e=[], tl=f["tl"]))
+                    sb = []
+                    assert top + len(children) <= FR, "frame stack overflow"
+                    for k, ch in enumerate(children):
+                        frames[top + k] = ch
+                    top += len(children)
+                    continue
+                L = Ltop
+                lim = WAVE - len(sb) if compact else WAVE
                 fcnt = [0] * WAVE
                 for lane in range(WAVE):
                     fi = top - 1 - lane
@@ -463,10 +489,10 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                         fcnt[lane] = frames[fi]["end"] - frames[fi]["cur"]
                 incl = [int(x) for x in np.cumsum(fcnt)]
                 excl = [incl[i] - fcnt[i] for i in range(WAVE)]
-                total = min(WAVE, incl[-1])
-                part = [fcnt[i] > 0 and excl[i] < WAVE for i in range(WAVE)]
+                total = min(lim, incl[-1])
+                part = [fcnt[i] > 0 and excl[i] < lim for i in range(WAVE)]
                 nparts = sum(part)
-                take = [min(fcnt[i], WAVE - excl[i]) if part[i] else 0 for i in range(WAVE)]
+                take = [min(fcnt[i], lim - excl[i]) if part[i] else 0 for i in range(WAVE)]
                 last_partial = any(part[i] and take[i] < fcnt[i] for i in range(WAVE))
                 sm = 0
                 for i in range(WAVE):
@@ -490,6 +516,7 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                 j = L + 1
                 kind = md["kind"][j]
                 children = []
+                cur = []  # survivors of this step (compaction)
                 for slot, f in items:
                     m = list(f["m"])
                     if kind == K_NEW_DST:
@@ -513,6 +540,10 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                     if tfn is not None:
                         assert tfn == g["tfirst"][md["tmode"][j + 1]][pos]
                     empty_next = tfn is not None and tfn > f["tl"]
+                    if compact:
+                        if not empty_next:
+                            cur.append((pos, m, f))
+                        continue
                     if j == last:
                         if enumerate_:
                             if cnt < cap:
@@ -529,6 +560,26 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                     ce = gallop_lb(L[0], L[2], L[3], f["pend"])
                     if ce > cb:
                         children.append(dict(cur=cb, end=ce, p=pos, pend=f["pend"], lvl=j, dir=d, m=m, e=e, tl=f["tl"]))
+                if compact:
+                    # buffered while few and more of this level follows, else
+                    # expanded in this step (nothing buffered)
+                    level_done = ntop == 0 or frames[ntop - 1]["lvl"] != L
+                    if sb or (len(cur) < sb_inline and not level_done):
+                        sb += cur
+                        sb_level = L
+                        assert len(sb) <= WAVE
+                        top = ntop
+                        stats["buffered_steps"] = stats.get("buffered_steps", 0) + 1
+                        continue
+                    for pos, m, f in cur:
+                        Lc = cand_start(g, md, j + 1, m, pos)
+                        if j == last:
+                            cnt += leaf_count(g, md, j + 1, m, Lc, pos, f["pend"], scan_max)
+                            continue
+                        ce = gallop_lb(Lc[0], Lc[2], Lc[3], f["pend"])
+                        if ce > Lc[2]:
+                            children.append(dict(cur=Lc[2], end=ce, p=pos, pend=f["pend"], lvl=j, dir=Lc[4], m=m,
+                                                 e=[], tl=f["tl"]))
                 assert ntop + len(children) <= FR, "frame stack overflow"
                 for k, ch in enumerate(children):
                     frames[ntop + k] = ch

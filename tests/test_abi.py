@@ -25,7 +25,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_version_and_no_device_ok():
     L = abi.lib()
-    assert L.tm_abi_version() == 2
+    assert L.tm_abi_version() == 3
     assert L.tm_device_count() >= 0
 
 

@@ -22,7 +22,8 @@ def test_model_matches_oracle():
         want = oracle.count(g, mot, delta, nthreads=1)
         scan = int(rng.choice([0, 2, 16]))
         got, _, st = km.run_wave(gv, md, delta, 0, g["E"] - M + 1, scan_max=scan,
-                                 chunk_roots=int(rng.choice([64, 128, 256])))
+                                 chunk_roots=int(rng.choice([64, 128, 256])),
+                                 sb_flush=int(rng.choice([0, 1, 32, 64])), sb_inline=int(rng.choice([1, 8, 32, 64])))
         assert got == want
         if want < 3000:
             total, rows = oracle.enum(g, mot, delta, 100000)
