@@ -9,8 +9,11 @@ log-uniform in [600, 604800] s) on the resident 64M-edge power-law graph.
           such launch over the whole query set
 
 Every batched count is checked bit-exact against the plain count of the same
-query.  Queries that exceed the per-query budget in the plain pass are listed
-and left out of both passes (reported in the JSON line).
+query.  The heaviest queries are left out of every pass: for the default set a
+fixed list (the 18 whose count took over 2 s when the workload was fixed, so
+that the query set does not change as the kernel gets faster); for any other
+set, or with --screen, the queries over --qlimit-ms in a screening pass.  The
+excluded indexes are reported in the JSON line.
 
 value = batched queries/s.  roofline: algorithmic bytes of the reference's own
 work (12 R + 8 X1 + 16 X2 + 16 P, counted exactly by the instrumented kernel)
@@ -28,6 +31,10 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+
+# The default set's queries whose count took over 2 s (reference work of about
+# 10^13 probes and more each): left out so the workload stays the same set.
+EXCLUDED_PL64M_256_SEED4 = (14, 17, 21, 22, 45, 52, 61, 64, 82, 87, 126, 132, 149, 181, 234, 235, 241, 244)
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
@@ -49,7 +56,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget (0 = skip)")
     ap.add_argument("--json-out", default="")
     ap.add_argument("--plain-only", action="store_true", help="stop after the plain pass (counter runs)")
-    ap.add_argument("--over", default="", help="comma-separated over-budget query indexes: skips the screening pass")
+    ap.add_argument("--over", default="", help="comma-separated excluded query indexes (skips the screening pass)")
+    ap.add_argument("--screen", action="store_true", help="screen the set against --qlimit-ms instead of the fixed list")
     ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned table (library defaults)")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
@@ -102,8 +110,12 @@ def main():
     # are exactly the kept queries.
     lim = abi.tuning(time_limit_ms=args.qlimit_ms)
     over = [int(x) for x in args.over.split(",") if x.strip()]
+    fixed = (args.graph, args.queries, args.seed) == ("pl64m", 256, 4) and not args.over and not args.screen
+    if fixed:
+        over = list(EXCLUDED_PL64M_256_SEED4)
+        log("excluded (fixed list of the default set): %s" % over)
     t = time.perf_counter()
-    for i, (m, d) in enumerate(motifs if not args.over else []):
+    for i, (m, d) in enumerate(motifs if not (args.over or fixed) else []):
         try:
             dg.batch_count([(m, d)], tune=lim)
         except abi.TMError as e:
@@ -225,9 +237,11 @@ def main():
         "dtype": "int32/u64",
         "data": "synthetic (seeded generator)",
         "config": {
-            "workload": "config4-batched: %d queries (%d kept, %d over %d ms budget), M=3..6 on <=4 nodes, "
+            "workload": "config4-batched: %d queries (%d kept, %d heaviest excluded: %s), M=3..6 on <=4 nodes, "
                         "delta log-uniform [600,604800]s, graph %s E=%d" % (
-                            len(motifs), len(keep), len(over), args.qlimit_ms, args.graph, hg.numEdges),
+                            len(motifs), len(keep), len(over),
+                            "fixed list, each over 2000 ms" if fixed else "over %d ms when screened" % args.qlimit_ms,
+                            args.graph, hg.numEdges),
             "plain_qps": round(plain_qps, 3),
             "batched_over_plain": round(batch_qps / plain_qps, 3),
             "plain_latency_ms_p50": round(float(np.percentile(plain_ms, 50)), 3),

@@ -21,13 +21,17 @@ import time
 
 from . import abi
 
-# register cap (5: 96 VGPRs, 6: 80 VGPRs), donation trigger, scan threshold
+# register cap (5: 96 VGPRs, 6: 80 VGPRs), donation trigger, scan threshold,
+# survivor compaction (the library default uses it where its LDS buffer costs
+# no resident wavefronts; the last three force it off or on)
 CANDIDATES = [
     dict(waves_per_simd_hint=5),
     dict(waves_per_simd_hint=6),
     dict(waves_per_simd_hint=5, steal_after_us=250),
-    dict(waves_per_simd_hint=6, steal_after_us=250),
     dict(waves_per_simd_hint=5, scan_max=32),
+    dict(waves_per_simd_hint=5, sb_flush=-1),
+    dict(waves_per_simd_hint=5, sb_flush=32, sb_inline=32),
+    dict(waves_per_simd_hint=5, sb_flush=48, sb_inline=48),
 ]
 
 

@@ -389,10 +389,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   unsigned long long cnt = 0;
-  unsigned long long st_steps = 0, st_items = 0, st_roots = 0, st_push = 0;  // uniform
+  unsigned st_steps = 0, st_roots = 0, st_push = 0;  // uniform, per wavefront
+  unsigned long long st_items = 0;
   unsigned st_leaves = 0;  // per lane
   unsigned long long rx1 = 0, rx2 = 0, rp = 0;  // reference work (INSTR only)
-  unsigned long long st_don = 0, st_take = 0, st_capexit = 0;
+  unsigned st_don = 0, st_take = 0, st_capexit = 0;
   unsigned long long st_work = 0;            // wall-clock ticks this wavefront held work
   long long work_t0 = (long long)wall_clock64();
   unsigned iter = 0;
@@ -413,6 +414,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   bool waiter = false;     // counted in sched->waiters
   int sbn = 0;             // survivors buffered (uniform)
   int sb_level = 0;        // level of the frames they came from
+  int steal_cd = 1;        // expansion steps until the next donation check
 
   // copy one frame between the LDS stack (slot w) and a pool record, one field per lane
   auto frame_field = [&](int f, int w) -> int * {
@@ -434,6 +436,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         if (lane == 0) atomicOr(&sched->error, 4);
         break;
       }
+      // enumeration cap reached: the subtrees in flight are dropped (their
+      // rows could not be written), so the launch ends as soon as every
+      // wavefront sees the flag instead of after finishing them
+      if (ENUM && top > 0 && uni(ld_rlx(args.queries[qcur].stop))) top = 0;
     }
     if (top == 0 && sbn == 0 && roots_done) {
       // ------------------------------------------ idle: take donated work
@@ -902,10 +908,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       __syncthreads();
     }  // flush
     // ---------------------------------------------- donate to idle wavefronts
-    // only a wavefront that has held its current task for steal_after_ticks
-    // donates: short tasks never pay for the exchange
-    if (top > 0 && (st_steps % (unsigned long long)args.steal_every) == 0 &&
-        (long long)wall_clock64() - task_t0 > args.steal_after_ticks) {
+    // every steal_every steps; only a wavefront that has held its current
+    // task for steal_after_ticks donates: short tasks never pay for the exchange
+    bool look = false;
+    if (top > 0 && --steal_cd <= 0) {
+      steal_cd = args.steal_every;
+      look = (long long)wall_clock64() - task_t0 > args.steal_after_ticks;
+    }
+    if (look) {
       // donate only against unmet demand: tickets drawn minus records reserved
       int want = 0;
       if (lane == 0) {
