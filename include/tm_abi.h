@@ -203,8 +203,8 @@ tm_status tm_set_class_tuning(tm_graph *g, int32_t cls, const tm_tuning *tune);
  * msrc/mdst[q * TM_MAX_MOTIF_EDGES ...], num_edges[q], deltas[q] and roots
  * [0, E - num_edges[q] + 1).  counts[q] receives the exact count, equal to
  * tm_count of the same query.  The queries are grouped by stack depth; each
- * group is one persistent launch of the batch kernel, and the groups run
- * concurrently on the graph's launch slots (TM_SLOTS streams).  No
+ * group is one persistent launch of the batch kernel with the whole GPU, the
+ * groups one after another.  No
  * reference counterpart: the reference runs one generated worker per query
  * (system/batch-query.py:12-19).
  */

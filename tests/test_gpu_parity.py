@@ -132,9 +132,9 @@ def test_pl8m_root_subranges_and_timeouts():
     dg.close()
 
 
-def test_pl8m_batch_classes_concurrent():
+def test_pl8m_batch_classes():
     """A batch mixing every shape class (3..6 edges, up to 4 and up to 8
-    nodes) runs as concurrent per-class launches on several streams; each
+    nodes) runs as one launch per class, one after another; each
     count equals the single-query count, and a root-range sample of each
     equals the oracle."""
     abi = _abi()
