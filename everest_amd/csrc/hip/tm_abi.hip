@@ -265,7 +265,17 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
     if (a < n && b < n) {
       md->kind[j] = TM_K_CLOSE;
       md->smode[j] = (a == pa && b == pb) ? TM_S_PAIR_SELF : TM_S_SEARCH;
-      md->tmode[j] = (a == pa && b == pb) ? TM_T_PAIR_SELF : ((a == pb && b == pa) ? TM_T_RECIP : TM_T_NONE);
+      // pruning rule: the pair's own next edge when edge j-1 has the same or
+      // the reversed pair; otherwise a node edge j shares with edge j-1 must
+      // have an edge of the right direction after p (the pair's edges are
+      // among them), e.g. a == pb: some out-edge of m[a] after p
+      if (a == pa && b == pb) md->tmode[j] = TM_T_PAIR_SELF;
+      else if (a == pb && b == pa) md->tmode[j] = TM_T_RECIP;
+      else if (a == pa) md->tmode[j] = TM_T_OUT_SELF;
+      else if (a == pb) md->tmode[j] = TM_T_OUT_NEXT;
+      else if (b == pb) md->tmode[j] = TM_T_IN_SELF;
+      else if (b == pa) md->tmode[j] = TM_T_IN_NEXT;
+      else md->tmode[j] = TM_T_NONE;
     } else if (a < n && b == n) {
       md->kind[j] = TM_K_NEW_DST;
       md->smode[j] = a == pa ? TM_S_OUT_SELF : (a == pb ? TM_S_OUT_NEXT : TM_S_SEARCH);

@@ -45,6 +45,25 @@ def motif_edges(pathToMotif):
     return [(u, v) for (u, v, _) in m.edges()]
 
 
+def close_rule(a, b, pa, pb):
+    """Pruning rule (TM_T_*) of a closing edge a -> b after edge pa -> pb:
+    the pair's own next edge (same or reversed pair), else the next edge of
+    the right direction at a node the two edges share, else none."""
+    if (a, b) == (pa, pb):
+        return 4
+    if (a, b) == (pb, pa):
+        return 5
+    if a == pa:
+        return 0
+    if a == pb:
+        return 1
+    if b == pb:
+        return 2
+    if b == pa:
+        return 3
+    return -1
+
+
 def motif_tables(edges):
     """Per-edge tables of the kernel spec, as the library builds them at run time."""
     M = len(edges)
@@ -64,7 +83,7 @@ def motif_tables(edges):
         if a < n and b < n:
             kind[j] = K_CLOSE
             smode[j] = S_PAIR_SELF if (a, b) == (pa, pb) else S_SEARCH
-            tmode[j] = 4 if (a, b) == (pa, pb) else (5 if (a, b) == (pb, pa) else -1)
+            tmode[j] = close_rule(a, b, pa, pb)
         elif a < n and b == n:
             kind[j] = K_NEW_DST
             smode[j] = S_OUT_SELF if a == pa else (S_OUT_NEXT if a == pb else S_SEARCH)

@@ -234,7 +234,9 @@ def build_motif(edges):
         if a < n and b < n:
             kind[j] = K_CLOSE
             smode[j] = S_PAIR_SELF if (a, b) == (pa, pb) else S_SEARCH
-            tmode[j] = 4 if (a, b) == (pa, pb) else (5 if (a, b) == (pb, pa) else -1)
+            # the pair's own next edge, else a shared node's next edge of the right direction
+            tmode[j] = next(r for r, c in ((4, (a, b) == (pa, pb)), (5, (a, b) == (pb, pa)), (0, a == pa),
+                                           (1, a == pb), (2, b == pb), (3, b == pa), (-1, True)) if c)
         elif a < n and b == n:
             kind[j] = K_NEW_DST
             smode[j] = S_OUT_SELF if a == pa else (S_OUT_NEXT if a == pb else S_SEARCH)
