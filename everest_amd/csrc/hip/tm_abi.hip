@@ -246,6 +246,18 @@ tm_status dev_alloc(tm_graph *g, void **p, size_t bytes) {
 }
 
 // Build the device descriptor of a motif given as renamed (src, dst) pairs.
+// TM_T_* rule that a later motif edge a -> b imposes at the graph edge matched
+// for the motif edge pa -> pb (relative to that edge's position), or TM_T_NONE
+static int rule_at(int pa, int pb, int a, int b) {
+  if (a == pa && b == pb) return TM_T_PAIR_SELF;
+  if (a == pb && b == pa) return TM_T_RECIP;
+  if (a == pa) return TM_T_OUT_SELF;
+  if (a == pb) return TM_T_OUT_NEXT;
+  if (b == pb) return TM_T_IN_SELF;
+  if (b == pa) return TM_T_IN_NEXT;
+  return TM_T_NONE;
+}
+
 tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_dev *md) {
   if (M < 2 || M > TM_MAXE) return fail(TM_ERR_MOTIF, "motif must have 2..8 edges");
   memset(md, 0, sizeof(*md));
@@ -269,13 +281,7 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
       // the reversed pair; otherwise a node edge j shares with edge j-1 must
       // have an edge of the right direction after p (the pair's edges are
       // among them), e.g. a == pb: some out-edge of m[a] after p
-      if (a == pa && b == pb) md->tmode[j] = TM_T_PAIR_SELF;
-      else if (a == pb && b == pa) md->tmode[j] = TM_T_RECIP;
-      else if (a == pa) md->tmode[j] = TM_T_OUT_SELF;
-      else if (a == pb) md->tmode[j] = TM_T_OUT_NEXT;
-      else if (b == pb) md->tmode[j] = TM_T_IN_SELF;
-      else if (b == pa) md->tmode[j] = TM_T_IN_NEXT;
-      else md->tmode[j] = TM_T_NONE;
+      md->tmode[j] = rule_at(pa, pb, a, b);
     } else if (a < n && b == n) {
       md->kind[j] = TM_K_NEW_DST;
       md->smode[j] = a == pa ? TM_S_OUT_SELF : (a == pb ? TM_S_OUT_NEXT : TM_S_SEARCH);
@@ -292,6 +298,20 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
     if (n > TM_MAXN) return fail(TM_ERR_MOTIF, "motif has more than 8 nodes");
   }
   md->n = n;
+  // second pruning rule per edge j: a later edge k >= j + 2 that meets a node
+  // of edge j needs that node's next edge of its direction after edge j's
+  // position inside the window (a pair rule preferred; none that repeats
+  // edge j + 1's own rule)
+  for (int j = 0; j < TM_MAXE; j++) md->xmode[j] = TM_T_NONE;
+  for (int j = 0; j + 2 < M; j++) {
+    int best = TM_T_NONE;
+    for (int k = j + 2; k < M && best != TM_T_PAIR_SELF; k++) {
+      const int r = rule_at(src[j], dst[j], src[k], dst[k]);
+      if (r == TM_T_NONE || r == md->tmode[j + 1]) continue;
+      if (best == TM_T_NONE || r == TM_T_PAIR_SELF || (r == TM_T_RECIP && best != TM_T_RECIP)) best = r;
+    }
+    md->xmode[j] = best;
+  }
   return TM_OK;
 }
 
@@ -426,7 +446,9 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
     qt[0] = qd.delta;
     qt[1] = qd.root_beg;
     qt[2] = qd.root_end;
-    qt[3] = qd.m.tmode[1];
+    // start rule of edge 1 and, outside the instrumented kernel (which
+    // follows the reference's search tree), the root's second rule
+    qt[3] = qd.m.tmode[1] | ((!instr && qd.m.xmode[0] >= 0) ? (qd.m.xmode[0] + 1) << TM_QT_XRULE_SHIFT : 0);
     s.h_prefix[q0 + i] = total_blocks;
     if (qd.root_end > qd.root_beg) {
       const long long nb = (qd.root_end - (qd.root_beg & ~3) + 255) / 256;

@@ -64,6 +64,22 @@ def close_rule(a, b, pa, pb):
     return -1
 
 
+def second_rule(edges, j, next_rule):
+    """Rule (TM_T_*) a later edge k >= j + 2 imposes at the graph edge matched
+    for edge j: a pair rule preferred, none that repeats next_rule."""
+    best = -1
+    pa, pb = edges[j]
+    for k in range(j + 2, len(edges)):
+        if best == 4:
+            break
+        r = close_rule(edges[k][0], edges[k][1], pa, pb)
+        if r < 0 or r == next_rule:
+            continue
+        if best < 0 or r == 4 or (r == 5 and best != 5):
+            best = r
+    return best
+
+
 def motif_tables(edges):
     """Per-edge tables of the kernel spec, as the library builds them at run time."""
     M = len(edges)
@@ -98,7 +114,10 @@ def motif_tables(edges):
             raise ValueError("motif is not prefix-connected")
         if n > MAXN:
             raise ValueError("motif has more than %d nodes" % MAXN)
-    return dict(M=M, N=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode)
+    xmode = [-1] * MAXE
+    for j in range(M - 2):
+        xmode[j] = second_rule(edges, j, tmode[j + 1])
+    return dict(M=M, N=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode, xmode=xmode)
 
 
 class KernelGen:
@@ -142,7 +161,7 @@ class KernelGen:
                     options=", ".join("%s=%s" % kv for kv in sorted(self.options.opts.items())),
                     src=join(a for a, _ in self.edges), dst=join(b for _, b in self.edges),
                     enum_cap=self.enum_NUM, kind=join(tb["kind"]), a=join(tb["a"]), b=join(tb["b"]),
-                    mapped=join(tb["mapped"]), smode=join(tb["smode"]), tmode=join(tb["tmode"]),
+                    mapped=join(tb["mapped"]), smode=join(tb["smode"]), tmode=join(tb["tmode"]), xmode=join(tb["xmode"]),
                     nmap=4 if tb["N"] <= 4 else 8)
         subs.update(self.tuning)
         return tmpl.substitute(subs)

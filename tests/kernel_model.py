@@ -131,7 +131,21 @@ def next_first_time(g, md, j, slot, direction, M):
 def root_passes(g, md, delta, r):
     """Root filter of the dispatch: the root's gap under edge 1's start rule
     is within delta (unsigned compare; self-loops carry 0xffffffff)."""
-    return delta >= 0 and g["rgap"][md["tmode"][1]][r] <= delta
+    ok = delta >= 0 and g["rgap"][md["tmode"][1]][r] <= delta
+    # the root's second rule (a later edge at a root node)
+    return ok and (md["xmode"][0] < 0 or g["rgap"][md["xmode"][0]][r] <= delta)
+
+
+def second_ok(g, md, j, slot, direction, tl):
+    """The candidate of edge j at `slot` passes edge j's second rule: the next
+    edge after its position at the node a later edge k >= j + 2 shares is in
+    the window."""
+    xm = md["xmode"][j] if j < len(md["xmode"]) else -1
+    if xm < 0:
+        return True
+    kind = md["kind"][j]
+    lst = L_OUT if kind == K_NEW_DST else (L_IN if kind == K_NEW_SRC else (L_INPAIR if direction else L_OUTPAIR))
+    return g["stf"][lst][xm][slot] <= tl
 
 
 STRIPES = 8
@@ -205,7 +219,9 @@ def batch_dispatch(g, queries, nwaves=5, per_claim=3, seed=0, instr=False):
             for lane in range(WAVE):
                 r0 = blk + 4 * lane
                 assert r0 % 4 == 0 and r0 + 3 < g["epad"]  # one aligned 16-byte load
-                gaps.append([g["rgap"][rule][r0 + i] if b <= r0 + i < e else 0xFFFFFFFF for i in range(4)])
+                xr = md["xmode"][0] if not instr else -1
+                gaps.append([(max(g["rgap"][rule][r0 + i], g["rgap"][xr][r0 + i]) if xr >= 0 else g["rgap"][rule][r0 + i])
+                             if b <= r0 + i < e else 0xFFFFFFFF for i in range(4)])
             near = any(d >= 0 and min(x) <= d for x in gaps)
             if not near and not instr:
                 continue
@@ -249,7 +265,23 @@ def build_motif(edges):
             n += 1
         else:
             raise ValueError("not prefix-connected")
-    return dict(M=M, n=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode)
+    # second rule per edge j: a later edge k >= j + 2 at a node of edge j (pair rules preferred)
+    def rule_at(pa, pb, a, b):
+        return next(r for r, c in ((4, (a, b) == (pa, pb)), (5, (a, b) == (pb, pa)), (0, a == pa), (1, a == pb),
+                                   (2, b == pb), (3, b == pa), (-1, True)) if c)
+    xmode = [-1] * M
+    for j in range(M - 2):
+        best = -1
+        for k in range(j + 2, M):
+            if best == 4:
+                break
+            r = rule_at(edges[j][0], edges[j][1], edges[k][0], edges[k][1])
+            if r < 0 or r == tmode[j + 1]:
+                continue
+            if best < 0 or r == 4 or (r == 5 and best != 5):
+                best = r
+        xmode[j] = best
+    return dict(M=M, n=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode, xmode=xmode)
 
 
 def lower_bound(a, lo, hi, x):
@@ -532,6 +564,7 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                     else:
                         pos = g["inPairPos"][slot] if f["dir"] else g["outPairPos"][slot]
                         valid = True
+                    valid = valid and (instr or second_ok(g, md, j, slot, f["dir"], f["tl"]))
                     if not valid:
                         continue
                     if instr and j + 1 < M:
@@ -785,6 +818,7 @@ class Wave:
             else:
                 pos = g["inPairPos"][slot] if f["dir"] else g["outPairPos"][slot]
                 valid = True
+            valid = valid and second_ok(g, md, j, slot, f["dir"], f["tl"])
             if not valid:
                 continue
             tfn = next_first_time(g, md, j, slot, f["dir"], M)
