@@ -298,19 +298,23 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
     if (n > TM_MAXN) return fail(TM_ERR_MOTIF, "motif has more than 8 nodes");
   }
   md->n = n;
-  // second pruning rule per edge j: a later edge k >= j + 2 that meets a node
-  // of edge j needs that node's next edge of its direction after edge j's
-  // position inside the window (a pair rule preferred; none that repeats
-  // edge j + 1's own rule)
-  for (int j = 0; j < TM_MAXE; j++) md->xmode[j] = TM_T_NONE;
+  // further pruning rules per edge j: every later edge k >= j + 2 that meets
+  // a node of edge j needs that node's next edge of its direction after edge
+  // j's position inside the window.  Rules implied by another are dropped
+  // (the pair's own next edge is an out-edge of its source and an in-edge of
+  // its destination; the reversed pair's likewise), and so is edge j + 1's.
   for (int j = 0; j + 2 < M; j++) {
-    int best = TM_T_NONE;
-    for (int k = j + 2; k < M && best != TM_T_PAIR_SELF; k++) {
+    int mask = 0;
+    for (int k = j + 2; k < M; k++) {
       const int r = rule_at(src[j], dst[j], src[k], dst[k]);
-      if (r == TM_T_NONE || r == md->tmode[j + 1]) continue;
-      if (best == TM_T_NONE || r == TM_T_PAIR_SELF || (r == TM_T_RECIP && best != TM_T_RECIP)) best = r;
+      if (r != TM_T_NONE) mask |= 1 << r;
     }
-    md->xmode[j] = best;
+    const int next = md->tmode[j + 1] >= 0 ? 1 << md->tmode[j + 1] : 0;
+    const int have = mask | next;
+    if (have & (1 << TM_T_PAIR_SELF)) mask &= ~((1 << TM_T_OUT_SELF) | (1 << TM_T_IN_SELF));
+    if (have & (1 << TM_T_RECIP)) mask &= ~((1 << TM_T_OUT_NEXT) | (1 << TM_T_IN_NEXT));
+    mask &= ~next;
+    md->xmask[j] = mask;
   }
   return TM_OK;
 }
@@ -448,7 +452,7 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
     qt[2] = qd.root_end;
     // start rule of edge 1 and, outside the instrumented kernel (which
     // follows the reference's search tree), the root's second rule
-    qt[3] = qd.m.tmode[1] | ((!instr && qd.m.xmode[0] >= 0) ? (qd.m.xmode[0] + 1) << TM_QT_XRULE_SHIFT : 0);
+    qt[3] = qd.m.tmode[1] | (instr ? 0 : qd.m.xmask[0] << TM_QT_XRULE_SHIFT);
     s.h_prefix[q0 + i] = total_blocks;
     if (qd.root_end > qd.root_beg) {
       const long long nb = (qd.root_end - (qd.root_beg & ~3) + 255) / 256;

@@ -168,7 +168,7 @@ struct DynSpec {
   static __device__ __forceinline__ int mapped(const tm_motif_dev &md, int j) { return md.mapped[j]; }
   static __device__ __forceinline__ int smode(const tm_motif_dev &md, int j) { return md.smode[j]; }
   static __device__ __forceinline__ int tmode(const tm_motif_dev &md, int j) { return md.tmode[j]; }
-  static __device__ __forceinline__ int xmode(const tm_motif_dev &md, int j) { return md.xmode[j]; }
+  static __device__ __forceinline__ int xmask(const tm_motif_dev &md, int j) { return md.xmask[j]; }
 };
 
 // element j of a compile-time table (j uniform: folds to scalar selects)
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   int quarter = 0;         // quarter of the block: this lane's root is blk + 4 lane + quarter
   int cq = 0;              // query of the current chunk, and its table entry:
   int cdelta = 0, cbeg = 0, cend = 0, crule = 0;
-  int cxrule = -1;         // the root's second gap rule (a later edge at a root node), or -1
+  int cxmask = 0;          // the root's further gap rules (later edges at the root's nodes), bit per rule
   uint4 bgap = make_uint4(0u, 0u, 0u, 0u);  // gaps of this lane's 4 roots of the block
   bool roots_done = false; // the root chunks of every query are handed out
   const int home_stripe = blockIdx.x & (TM_STRIPES - 1);
@@ -583,14 +583,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             const int4 qt = *(const int4 *)(args.qtab + 4 * cq);
             cdelta = qt.x; cbeg = qt.y; cend = qt.z;
             crule = qt.w & ((1 << TM_QT_XRULE_SHIFT) - 1);
-            cxrule = (qt.w >> TM_QT_XRULE_SHIFT) - 1;
+            cxmask = qt.w >> TM_QT_XRULE_SHIFT;
           }
           blk = (cbeg & ~3) + (int)(b - qb0) * TM_BLOCK;
           if (ENUM && ld_rlx(args.queries[cq].stop)) { blk = -1; continue; }  // enumeration cap reached
           const int r0 = blk + 4 * lane;
           uint4 x = *(const uint4 *)(g.rgap + (size_t)crule * (size_t)g.epad + r0);
-          if (cxrule >= 0) {  // a root passes only if both rules' first edges are in the window
-            const uint4 y = *(const uint4 *)(g.rgap + (size_t)cxrule * (size_t)g.epad + r0);
+          for (int rr = 0; rr < TM_NTFIRST; rr++) {  // a root passes only if every rule's next edge is in the window
+            if (!((cxmask >> rr) & 1)) continue;
+            const uint4 y = *(const uint4 *)(g.rgap + (size_t)rr * (size_t)g.epad + r0);
             x.x = max(x.x, y.x); x.y = max(x.y, y.y); x.z = max(x.z, y.z); x.w = max(x.w, y.w);
           }
           if (r0 < cbeg || r0 >= cend) x.x = ~0u;
@@ -767,12 +768,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       const int lst = kind == TM_K_NEW_DST ? TM_L_OUT
                       : (kind == TM_K_NEW_SRC ? TM_L_IN : (dir ? TM_L_INPAIR : TM_L_OUTPAIR));
       if (tmn >= 0) tfn = g.stf[(size_t)(lst * TM_NTFIRST + tmn) * (size_t)g.E + slot];
-      // second rule: a later edge (k >= j + 2) at a node of this edge needs
-      // that node's next edge after this position in the window (not in the
-      // instrumented kernel, which follows the reference's search tree)
-      const int xm = INSTR ? -1 : SP::xmode(md, j);
-      int tx = 0;
-      if (xm >= 0) tx = g.stf[(size_t)(lst * TM_NTFIRST + xm) * (size_t)g.E + slot];
+      // further rules: each later edge (k >= j + 2) at a node of this edge
+      // needs that node's next edge after this position in the window (not
+      // in the instrumented kernel, which follows the reference's search tree)
+      const int xm = INSTR ? 0 : SP::xmask(md, j);
+      int tx = -0x7fffffff - 1;
+#pragma unroll
+      for (int rr = 0; rr < TM_NTFIRST; rr++)
+        if ((xm >> rr) & 1) tx = max(tx, g.stf[(size_t)(lst * TM_NTFIRST + rr) * (size_t)g.E + slot]);
       if (kind == TM_K_NEW_DST) {
         pos = g.outPos[slot]; node = g.outNbr[slot];
         valid = !in_first(m, SP::mapped(md, j), node);

@@ -38,7 +38,7 @@ enum {
   TM_T_RECIP = 5      // next edge dst p -> src p
 };
 #define TM_NTFIRST 6
-#define TM_QT_XRULE_SHIFT 8  // query table rule word: (root's second gap rule + 1) << TM_QT_XRULE_SHIFT
+#define TM_QT_XRULE_SHIFT 8  // query table rule word: edge 1's rule | the root's further rules << TM_QT_XRULE_SHIFT
 
 typedef struct { int u, v, t; } tm_edge;  // reference TemporalEdge (include/data.h:27-29)
 
@@ -51,8 +51,8 @@ typedef struct {
   int mapped[TM_MAXE];  // nodes mapped before edge j (a new node gets this index)
   int smode[TM_MAXE];   // TM_S_* start rule of edge j's candidate list
   int tmode[TM_MAXE];   // TM_T_* first-candidate time array of edge j (or TM_T_NONE)
-  int xmode[TM_MAXE];   // TM_T_* rule for a later edge k >= j + 2 at a node edge j matched
-                        // (its next such edge after edge j's position must be in the window), or TM_T_NONE
+  int xmask[TM_MAXE];   // TM_T_* rules (bit r) that later edges k >= j + 2 impose at the nodes of
+                        // edge j's match: each rule's next edge after that position must be in the window
 } tm_motif_dev;
 
 // Device-resident graph: reference arrays plus the indexes built at upload.

@@ -64,20 +64,23 @@ def close_rule(a, b, pa, pb):
     return -1
 
 
-def second_rule(edges, j, next_rule):
-    """Rule (TM_T_*) a later edge k >= j + 2 imposes at the graph edge matched
-    for edge j: a pair rule preferred, none that repeats next_rule."""
-    best = -1
+def further_rules(edges, j, next_rule):
+    """Rules (TM_T_* bits) that later edges k >= j + 2 impose at the graph edge
+    matched for edge j, without those implied by another or equal to
+    next_rule (edge j + 1's own)."""
     pa, pb = edges[j]
+    mask = 0
     for k in range(j + 2, len(edges)):
-        if best == 4:
-            break
         r = close_rule(edges[k][0], edges[k][1], pa, pb)
-        if r < 0 or r == next_rule:
-            continue
-        if best < 0 or r == 4 or (r == 5 and best != 5):
-            best = r
-    return best
+        if r >= 0:
+            mask |= 1 << r
+    nxt = (1 << next_rule) if next_rule >= 0 else 0
+    have = mask | nxt
+    if have & (1 << 4):
+        mask &= ~((1 << 0) | (1 << 2))
+    if have & (1 << 5):
+        mask &= ~((1 << 1) | (1 << 3))
+    return mask & ~nxt
 
 
 def motif_tables(edges):
@@ -114,10 +117,10 @@ def motif_tables(edges):
             raise ValueError("motif is not prefix-connected")
         if n > MAXN:
             raise ValueError("motif has more than %d nodes" % MAXN)
-    xmode = [-1] * MAXE
+    xmask = [0] * MAXE
     for j in range(M - 2):
-        xmode[j] = second_rule(edges, j, tmode[j + 1])
-    return dict(M=M, N=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode, xmode=xmode)
+        xmask[j] = further_rules(edges, j, tmode[j + 1])
+    return dict(M=M, N=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode, xmask=xmask)
 
 
 class KernelGen:
@@ -161,7 +164,7 @@ class KernelGen:
                     options=", ".join("%s=%s" % kv for kv in sorted(self.options.opts.items())),
                     src=join(a for a, _ in self.edges), dst=join(b for _, b in self.edges),
                     enum_cap=self.enum_NUM, kind=join(tb["kind"]), a=join(tb["a"]), b=join(tb["b"]),
-                    mapped=join(tb["mapped"]), smode=join(tb["smode"]), tmode=join(tb["tmode"]), xmode=join(tb["xmode"]),
+                    mapped=join(tb["mapped"]), smode=join(tb["smode"]), tmode=join(tb["tmode"]), xmask=join(tb["xmask"]),
                     nmap=4 if tb["N"] <= 4 else 8)
         subs.update(self.tuning)
         return tmpl.substitute(subs)

@@ -132,20 +132,18 @@ def root_passes(g, md, delta, r):
     """Root filter of the dispatch: the root's gap under edge 1's start rule
     is within delta (unsigned compare; self-loops carry 0xffffffff)."""
     ok = delta >= 0 and g["rgap"][md["tmode"][1]][r] <= delta
-    # the root's second rule (a later edge at a root node)
-    return ok and (md["xmode"][0] < 0 or g["rgap"][md["xmode"][0]][r] <= delta)
+    # the root's further rules (later edges at the root's nodes)
+    return ok and all(g["rgap"][x][r] <= delta for x in range(6) if (md["xmask"][0] >> x) & 1)
 
 
 def second_ok(g, md, j, slot, direction, tl):
-    """The candidate of edge j at `slot` passes edge j's second rule: the next
-    edge after its position at the node a later edge k >= j + 2 shares is in
-    the window."""
-    xm = md["xmode"][j] if j < len(md["xmode"]) else -1
-    if xm < 0:
-        return True
+    """The candidate of edge j at `slot` passes edge j's further rules: for
+    each later edge k >= j + 2 meeting a node of edge j, that node's next edge
+    of the right direction after the candidate's position is in the window."""
+    xm = md["xmask"][j] if j < len(md["xmask"]) else 0
     kind = md["kind"][j]
     lst = L_OUT if kind == K_NEW_DST else (L_IN if kind == K_NEW_SRC else (L_INPAIR if direction else L_OUTPAIR))
-    return g["stf"][lst][xm][slot] <= tl
+    return all(g["stf"][lst][x][slot] <= tl for x in range(6) if (xm >> x) & 1)
 
 
 STRIPES = 8
@@ -219,8 +217,8 @@ def batch_dispatch(g, queries, nwaves=5, per_claim=3, seed=0, instr=False):
             for lane in range(WAVE):
                 r0 = blk + 4 * lane
                 assert r0 % 4 == 0 and r0 + 3 < g["epad"]  # one aligned 16-byte load
-                xr = md["xmode"][0] if not instr else -1
-                gaps.append([(max(g["rgap"][rule][r0 + i], g["rgap"][xr][r0 + i]) if xr >= 0 else g["rgap"][rule][r0 + i])
+                xm = md["xmask"][0] if not instr else 0
+                gaps.append([max([g["rgap"][rule][r0 + i]] + [g["rgap"][x][r0 + i] for x in range(6) if (xm >> x) & 1])
                              if b <= r0 + i < e else 0xFFFFFFFF for i in range(4)])
             near = any(d >= 0 and min(x) <= d for x in gaps)
             if not near and not instr:
@@ -269,19 +267,20 @@ def build_motif(edges):
     def rule_at(pa, pb, a, b):
         return next(r for r, c in ((4, (a, b) == (pa, pb)), (5, (a, b) == (pb, pa)), (0, a == pa), (1, a == pb),
                                    (2, b == pb), (3, b == pa), (-1, True)) if c)
-    xmode = [-1] * M
+    xmask = [0] * M
     for j in range(M - 2):
-        best = -1
+        mask = 0
         for k in range(j + 2, M):
-            if best == 4:
-                break
             r = rule_at(edges[j][0], edges[j][1], edges[k][0], edges[k][1])
-            if r < 0 or r == tmode[j + 1]:
-                continue
-            if best < 0 or r == 4 or (r == 5 and best != 5):
-                best = r
-        xmode[j] = best
-    return dict(M=M, n=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode, xmode=xmode)
+            if r >= 0:
+                mask |= 1 << r
+        nxt = (1 << tmode[j + 1]) if tmode[j + 1] >= 0 else 0
+        if (mask | nxt) & (1 << 4):
+            mask &= ~0b101   # pair rule implies the source's next out- and destination's next in-edge
+        if (mask | nxt) & (1 << 5):
+            mask &= ~0b1010  # reversed pair implies the other two
+        xmask[j] = mask & ~nxt
+    return dict(M=M, n=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode, xmask=xmask)
 
 
 def lower_bound(a, lo, hi, x):
