@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--screen", action="store_true", help="screen the set against --qlimit-ms instead of the fixed list")
     ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned table (library defaults)")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving run (0 = skip)")
-    ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
+    ap.add_argument("--serve-qps", type=float, default=200.0, help="Poisson arrival rate of the serving run")
     ap.add_argument("--serve-enum-frac", type=float, default=0.2, help="share of enumerate requests")
     ap.add_argument("--serve-enum-cap", type=int, default=1000000, help="row cap of enumerate requests")
     ap.add_argument("--calibrate-copy", type=int, default=0,
