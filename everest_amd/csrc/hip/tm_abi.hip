@@ -993,8 +993,12 @@ static int shape_class(int M, int n) { return std::max(1, M - 2) * 2 + (n <= 4 ?
 
 // The graph's tuning table: a call without launch parameters of its own uses
 // the entry of its query's shape class, if one was set.
+// The launch parameters of a call: its own, unless it brings none (a null
+// pointer or every field 0, both meaning "the library's choice"), in which
+// case the tuning installed for the motif's shape class, if any.
 static const tm_tuning *class_tuning(tm_graph *g, int cls, const tm_tuning *tune) {
-  if (tune) return tune;
+  static const tm_tuning none{};
+  if (tune && memcmp(tune, &none, sizeof(none)) != 0) return tune;
   if (cls >= 0 && cls < TM_NCLASSES && g->class_set[cls]) return &g->class_tune[cls];
   return nullptr;
 }

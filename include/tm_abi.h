@@ -191,7 +191,7 @@ tm_status tm_enum_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, co
  * motif's class is 2 * max(1, M - 2) + (nodes > 4): the LDS stack depth and
  * node-map width of its launch, which set its occupancy.  tm_count,
  * tm_batch_count and the server use the class's entry when they are called
- * without launch parameters (tune == NULL).  tune == NULL in
+ * without launch parameters (tune == NULL or every field 0).  tune == NULL in
  * tm_set_class_tuning clears the entry.  No reference counterpart.
  */
 #define TM_NCLASSES 16
