@@ -322,7 +322,7 @@ def serve(dg, motifs, keep, counts, plain_ms, args):
     plain_span = done.max() / 1e3
 
     log("serving: %d requests (%d enumerate, cap %d) at %.0f/s offered" % (n, int(enum.sum()), cap, args.serve_qps))
-    srv = abi.Server(dg, slots=4, max_batch=256)
+    srv = abi.Server(dg, max_batch=256)  # the library's slot count
     tickets = [None] * n
     t0 = time.perf_counter()
 

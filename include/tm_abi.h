@@ -246,7 +246,7 @@ tm_status tm_batch_free(tm_batch *b);
 typedef struct tm_server tm_server;
 
 typedef struct tm_server_config {
-  int32_t slots;        /* launch slots used, 1 .. 4 (0: 4) */
+  int32_t slots;        /* launch slots used, 1 .. 4 (0: 2) */
   int32_t max_batch;    /* count requests per launch, 1 .. 512 (0: 256) */
   int64_t arena_bytes;  /* device memory for enumeration rows in flight (0: 1 GiB) */
   tm_tuning tune;       /* launch parameters of every launch */
