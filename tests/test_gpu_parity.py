@@ -173,6 +173,47 @@ def test_pl8m_pruned_equals_unpruned():
     dg.close()
 
 
+def test_pl64m_full_size_properties():
+    """At the benchmark's full size (64M edges, the benchmark's own queries):
+    counts are additive over root ranges, the batch equals the single-query
+    counts, the pruned counts equal the unpruned instrumented kernel's on a
+    1M-root range, and the oracle agrees on a 20,000-root sample."""
+    abi = _abi()
+    from everest_amd import core
+    from everest_amd.gen import graphs, queries
+    U, V, T = graphs.make("pl64m")
+    el = core.data.EdgeListLoader.fromArrays("pl64m", U, V, T)
+    hg = core.data.GraphDataLoader(el, core.data.FeatureLoader(el, "")).createGraphData()
+    arrs = {k: hg.array(k) for k in ("Eg", "inEdgesV", "outEdgesV", "inEdgesR", "outEdgesR", "Euid")}
+    g = dict(E=hg.numEdges, Eg=arrs["Eg"].reshape(-1, 3), inV=arrs["inEdgesV"], outV=arrs["outEdgesV"],
+             inR=arrs["inEdgesR"], outR=arrs["outEdgesR"], Euid=arrs["Euid"])
+    dg = abi.DeviceGraph(arrs["Eg"], arrs["inEdgesV"], arrs["outEdgesV"], arrs["inEdgesR"], arrs["outEdgesR"],
+                         arrs["Euid"])
+    E = hg.numEdges
+    qs = queries.batch_queries(256, seed=4)
+    pick = [0, 3, 230, 169, 114, 220, 5, 106]  # benchmark queries of 3-6 edges, each under 0.3 s on the GPU
+    motifs = [(queries.as_motif(qs[i][0]), qs[i][1]) for i in pick]
+    full = []
+    for i, (mot, d) in zip(pick, motifs):
+        say("pl64m query", i, qs[i][0], "delta", d)
+        c = dg.count(mot, d)
+        cuts = [0, E // 3, 2 * E // 3, -1]
+        parts = sum(dg.count(mot, d, cuts[k], cuts[k + 1]) for k in range(3))
+        assert c == parts, (i, c, parts)
+        beg = 20_000_000
+        got = dg.count(mot, d, beg, beg + 1_000_000)
+        want, _ = dg.count_work(mot, d, beg, beg + 1_000_000)
+        assert got == want, (i, got, want)
+        full.append(c)
+    assert dg.batch_count(motifs) == full
+    for i, (mot, d) in zip(pick[-3:], motifs[-3:]):  # samples with matches (10^2 to 10^6)
+        beg = 10_000_000
+        want = oracle.count(g, mot, d, beg, beg + 20_000, nthreads=8)
+        assert want > 0 and dg.count(mot, d, beg, beg + 20_000) == want, i
+    say("pl64m counts", full)
+    dg.close()
+
+
 def test_pl8m_batch_classes():
     """A batch mixing every shape class (3..6 edges, up to 4 and up to 8
     nodes) runs as one launch per class, one after another; each
