@@ -398,7 +398,7 @@ def cpu_baseline(hg, arrs, motifs, keep, counts, dg, budget_s):
     threads = min(16, os.cpu_count() or 1)
     rng = np.random.default_rng(17)
     E = hg.numEdges
-    span = 1 << 16
+    span = 1 << 18  # about 10 s of CPU work over the kept queries on 16 threads
     est_full_s, n, checked, t_used = 0.0, 0, 0, 0.0
     for i in keep:
         if t_used > budget_s:
