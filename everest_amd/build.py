@@ -4,6 +4,7 @@
   libeverest_core.so  g++: loader, motif reader, runtime (include/everest/data.h)
   corelib*.so         g++ + pybind11: Python module "corelib"
   libTM.so            g++: stock worker plugin (getWorker entry point)
+  libCSR.so           hipcc: hand-written worker plugin reading DeviceJobData's device arrays
 
 Run: python -m everest_amd.build   (or __graft_entry__.build()).
 """
@@ -47,12 +48,24 @@ def ext_suffix():
     return sysconfig.get_config_var("EXT_SUFFIX") or ".so"
 
 
+HIP_UNITS = ("tm_abi.hip", "tm_server.hip", "tm_batch.hip", "tm_kv_n4_single.hip", "tm_kv_n4_multi.hip",
+             "tm_kv_n4_aux.hip", "tm_kv_n8_single.hip", "tm_kv_n8_multi.hip", "tm_kv_n8_aux.hip")
+
+
 def build_hip():
+    """Each translation unit compiled on its own (in parallel), then linked."""
     out = os.path.join(LIB, "libtm_hip.so")
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-I" + INC, "-I" + HIP_SRC,
-                                os.path.join(HIP_SRC, "tm_abi.hip"), os.path.join(HIP_SRC, "tm_server.hip"),
-                                os.path.join(HIP_SRC, "tm_batch.hip"),
-                                "-pthread", "-o", out])
+    objdir = os.path.join(LIB, "obj")
+    os.makedirs(objdir, exist_ok=True)
+
+    def unit(src):
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC] + HIP_FLAGS + ["-c", "-I" + INC, "-I" + HIP_SRC, os.path.join(HIP_SRC, src), "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=min(len(HIP_UNITS), os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(unit, HIP_UNITS))
+    _run([HIPCC] + HIP_FLAGS + ["-shared"] + objs + ["-pthread", "-o", out])
     return out
 
 
@@ -79,6 +92,14 @@ def build_plugin():
     return out
 
 
+def build_csr_plugin():
+    """Hand-written worker plugin bound to the reference's DeviceJobData fields."""
+    out = os.path.join(LIB, "libCSR.so")
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-I" + INC, os.path.join(PKG, "csrc", "plugins", "worker_csr.hip"),
+                                "-L" + LIB, "-leverest_core", "-ltm_hip"] + RPATH + ["-o", out])
+    return out
+
+
 def build_oracle():
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "libtm_oracle.so"])
     if os.path.isdir("/root/reference"):
@@ -93,11 +114,13 @@ def build_all(verbose=True):
         f_hip.result()
         f_or.result()
     build_core()
-    with ThreadPoolExecutor(max_workers=2) as ex:
+    with ThreadPoolExecutor(max_workers=3) as ex:
         a = ex.submit(build_py)
         b = ex.submit(build_plugin)
+        c = ex.submit(build_csr_plugin)
         a.result()
         b.result()
+        c.result()
     if verbose:
         print("built:", ", ".join(sorted(os.listdir(LIB))))
 

@@ -40,13 +40,13 @@ static py::array_t<int> ints_of_src(const data::SourceMemory *m) {
 }
 
 PYBIND11_MODULE(corelib, m) {
-  m.doc() = "Libraries for the temporal pattern mining system (MI355X build)";
+  m.doc() = "Temporal-motif query core: loaders, resident graphs, worker plugins and executions (MI355X)";
 
   auto d = m.def_submodule("data");
   py::bind_vector<data::HostGraphDataList>(d, "HostGraphDataList");
   py::bind_vector<data::SubPartitionList>(d, "SubPartitionList");
 
-  py::class_<data::EdgeListLoader>(d, "EdgeListLoader", "A dataloader that loads edge list")
+  py::class_<data::EdgeListLoader>(d, "EdgeListLoader", "Edge-list file (u v t per line) parsed and stably sorted by time")
       .def(py::init<std::filesystem::path>())
       .def_static(
           "fromArrays",
@@ -64,16 +64,16 @@ PYBIND11_MODULE(corelib, m) {
       .def("V", [](const data::EdgeListLoader &e) { return ints_of_src(e.getVPtr()); })
       .def("T", [](const data::EdgeListLoader &e) { return ints_of_src(e.getTPtr()); })
       .def("IDS", [](const data::EdgeListLoader &e) { return ints_of_src(e.getIDSPtr()); })
-      .def("__repr__", [](const data::EdgeListLoader &e) { return "EdgeListLoader Object <" + e.name() + ">"; });
+      .def("__repr__", [](const data::EdgeListLoader &e) { return "<EdgeListLoader " + e.name() + ">"; });
 
-  py::class_<data::FeatureLoader>(d, "FeatureLoader", "A feature Loader that loads random features to the graph")
+  py::class_<data::FeatureLoader>(d, "FeatureLoader", "Edge and vertex labels for a loaded edge list")
       .def(py::init<const data::EdgeListLoader *, std::filesystem::path>(), py::keep_alive<1, 2>())
       .def("name", &data::FeatureLoader::name)
       .def("edgeFeaturesLength", &data::FeatureLoader::edgeFeaturesLength)
       .def("verticesFeaturesLength", &data::FeatureLoader::verticesFeaturesLength)
-      .def("__repr__", [](const data::FeatureLoader &e) { return "FeatureLoader Object <" + e.name() + ">"; });
+      .def("__repr__", [](const data::FeatureLoader &e) { return "<FeatureLoader " + e.name() + ">"; });
 
-  py::class_<data::HostGraphData>(d, "HostGraphData", "The graph data for a subset of edge list")
+  py::class_<data::HostGraphData>(d, "HostGraphData", "Renamed, CSR-indexed host arrays of an edge range")
       .def_readonly("name", &data::HostGraphData::name)
       .def_readonly("numEdges", &data::HostGraphData::numEdges)
       .def_readonly("numVertices", &data::HostGraphData::numVertices)
@@ -94,7 +94,7 @@ PYBIND11_MODULE(corelib, m) {
            })
       .def("__repr__", [](const data::HostGraphData &e) { return repr("<HostGraphData>\n", e); });
 
-  py::class_<data::SubPartition>(d, "SubPartition", "A sub-partition of edge list")
+  py::class_<data::SubPartition>(d, "SubPartition", "Edge range [beg, end) of one time partition")
       .def_readonly("beg", &data::SubPartition::beg)
       .def_readonly("end", &data::SubPartition::end)
       .def_readonly("n", &data::SubPartition::n)
@@ -107,7 +107,7 @@ PYBIND11_MODULE(corelib, m) {
   d.def("removeMinor", &data::removeMinor);
   d.def("includeMinor", &data::includeMinor);
 
-  py::class_<data::GraphDataLoader>(d, "GraphDataLoader", "A dataloader that loads graph data")
+  py::class_<data::GraphDataLoader>(d, "GraphDataLoader", "Builds HostGraphData (renaming, adjacency, caches) from a loaded edge list")
       .def(py::init<const data::EdgeListLoader *, const data::FeatureLoader *>(), py::keep_alive<1, 2>(),
            py::keep_alive<1, 3>())
       .def("createGraphData", &data::GraphDataLoader::createGraphData, py::arg("beg") = 0, py::arg("end") = 0,
@@ -117,10 +117,10 @@ PYBIND11_MODULE(corelib, m) {
       .def("createPartitionsDataMinor", &data::GraphDataLoader::createPartitionsDataMinor)
       .def("__repr__", [](const data::GraphDataLoader &e) { return "<GraphDataLoader: " + e.name() + ">"; });
 
-  py::class_<data::DeviceGraphData>(d, "DeviceGraphData", "The graph data on a device")
+  py::class_<data::DeviceGraphData>(d, "DeviceGraphData", "Host graph resident on one GPU")
       .def("__repr__", [](const data::DeviceGraphData &e) { return repr("<DeviceGraphData>\n", e); });
 
-  py::class_<data::GraphDataManager>(d, "GraphDataManager", "A class that move data asynchronously over multigpus")
+  py::class_<data::GraphDataManager>(d, "GraphDataManager", "Keeps host graphs resident on the GPUs it manages")
       .def("allocBatch", &data::GraphDataManager::allocBatch)
       .def("alloc", &data::GraphDataManager::alloc)
       .def("moveAsync", &data::GraphDataManager::moveAsync)
@@ -129,7 +129,7 @@ PYBIND11_MODULE(corelib, m) {
 
   d.def("MakeGraphDataManager", &data::MakeGraphDataManager);
 
-  py::class_<data::HostMotifData>(d, "HostMotifData", "Motif Data on the host")
+  py::class_<data::HostMotifData>(d, "HostMotifData", "Motif file read, nodes renamed, per-edge expansion info")
       .def(py::init<const std::filesystem::path &>())
       .def_readonly("name", &data::HostMotifData::name)
       .def_readonly("numEdges", &data::HostMotifData::numEdges)
@@ -148,11 +148,11 @@ PYBIND11_MODULE(corelib, m) {
            })
       .def("__repr__", [](const data::HostMotifData &e) { return repr("<HostMotifData>\n", e); });
 
-  py::class_<data::DeviceJobData>(d, "DeviceJobData", "Job Data on the Device")
+  py::class_<data::DeviceJobData>(d, "DeviceJobData", "A motif bound to a resident graph")
       .def(py::init<const data::DeviceGraphData &, const data::HostMotifData &>())
       .def("__repr__", [](const data::DeviceJobData &e) { return repr("<DeviceJobData>\n", e); });
 
-  py::class_<data::MineJob>(d, "MineJob", "A Job, contains everything to let a worker to mine")
+  py::class_<data::MineJob>(d, "MineJob", "Query job: graph + motif, delta and root range [beg, end)")
       .def(py::init<const data::DeviceJobData *, int, int, int>(), py::arg("_data"), py::arg("_delta"),
            py::arg("_beg") = 0, py::arg("_end") = -1, py::keep_alive<1, 2>())
       .def("getWork", &data::MineJob::getWork)
@@ -162,7 +162,7 @@ PYBIND11_MODULE(corelib, m) {
                std::to_string(e.beg) + ", " + std::to_string(e.end) + ">";
       });
 
-  py::class_<GPUWorker>(m, "GPUWorker", "GPU worker class")
+  py::class_<GPUWorker>(m, "GPUWorker", "Motif-mining worker of one GPU (plugin object)")
       .def("run", &GPUWorker::run, py::call_guard<py::gil_scoped_release>())
       .def("take", &GPUWorker::take, py::keep_alive<1, 2>())
       .def("update_job", &GPUWorker::update_job)
@@ -175,14 +175,14 @@ PYBIND11_MODULE(corelib, m) {
       .def_readonly("type_", &GPUWorker::type_)
       .def("__repr__", [](const GPUWorker &w) { return repr("<GPUWorker>\n", w); });
 
-  py::class_<GPUWorkerLibManager>(m, "GPUWorkerLibManager", "to manager GPU worker libraries")
+  py::class_<GPUWorkerLibManager>(m, "GPUWorkerLibManager", "Opens worker plugin libraries and constructs their workers")
       .def(py::init<>())
       .def("openLib", &GPUWorkerLibManager::openLib)
       .def("unloadLib", &GPUWorkerLibManager::unloadLib)
       .def("construct", &GPUWorkerLibManager::construct)
       .def("__repr__", [](const GPUWorkerLibManager &e) { return repr("<GPUWorkerLibManager>\n", e); });
 
-  py::class_<SingleGPUExecution>(m, "SingleGPUExecution", "SingleGPUExecution uses a single GPU to do the mining")
+  py::class_<SingleGPUExecution>(m, "SingleGPUExecution", "Runs one worker over the whole root range on one GPU")
       .def(py::init<const data::GraphDataManager *, int>(), py::arg("dataManager"), py::arg("device") = 0,
            py::keep_alive<1, 2>())
       .def("setHostGraph", &SingleGPUExecution::setHostGraph, py::arg("hostGraph"), py::arg("n") = -1,
@@ -190,9 +190,9 @@ PYBIND11_MODULE(corelib, m) {
       .def("setHostMotif", &SingleGPUExecution::setHostMotif, py::keep_alive<1, 2>())
       .def("setWorker", &SingleGPUExecution::setWorker, py::keep_alive<1, 2>())
       .def("run", &SingleGPUExecution::run, py::call_guard<py::gil_scoped_release>())
-      .def("__repr__", [](const SingleGPUExecution &) { return std::string("<SingleGPUExeuction>"); });
+      .def("__repr__", [](const SingleGPUExecution &) { return std::string("<SingleGPUExecution: one device>"); });
 
-  py::class_<SingleGPUExecutionDiv>(m, "SingleGPUExecutionDiv", "SingleGPUExecutionDiv uses a single GPU to do the mining")
+  py::class_<SingleGPUExecutionDiv>(m, "SingleGPUExecutionDiv", "Runs one worker over the root range split into div parts")
       .def(py::init<const data::GraphDataManager *, int, int>(), py::arg("dataManager"), py::arg("device") = 0,
            py::arg("div") = 64, py::keep_alive<1, 2>())
       .def("setHostGraph", &SingleGPUExecutionDiv::setHostGraph, py::arg("hostGraph"), py::arg("n") = -1,
@@ -200,9 +200,9 @@ PYBIND11_MODULE(corelib, m) {
       .def("setHostMotif", &SingleGPUExecutionDiv::setHostMotif, py::keep_alive<1, 2>())
       .def("setWorker", &SingleGPUExecutionDiv::setWorker, py::keep_alive<1, 2>())
       .def("run", &SingleGPUExecutionDiv::run, py::call_guard<py::gil_scoped_release>())
-      .def("__repr__", [](const SingleGPUExecutionDiv &) { return std::string("<SingleGPUExeuction>"); });
+      .def("__repr__", [](const SingleGPUExecutionDiv &) { return std::string("<SingleGPUExecution: one device>"); });
 
-  py::class_<SingleGPUExecutionPause>(m, "SingleGPUExecutionPause", "SingleGPUExecutionPause uses a single GPU to do the mining")
+  py::class_<SingleGPUExecutionPause>(m, "SingleGPUExecutionPause", "Runs one worker on one GPU (pausing is not needed here)")
       .def(py::init<const data::GraphDataManager *, int>(), py::arg("dataManager"), py::arg("device") = 0,
            py::keep_alive<1, 2>())
       .def("setHostGraph", &SingleGPUExecutionPause::setHostGraph, py::arg("hostGraph"), py::arg("n") = -1,
@@ -210,7 +210,7 @@ PYBIND11_MODULE(corelib, m) {
       .def("setHostMotif", &SingleGPUExecutionPause::setHostMotif, py::keep_alive<1, 2>())
       .def("setWorker", &SingleGPUExecutionPause::setWorker, py::keep_alive<1, 2>())
       .def("run", &SingleGPUExecutionPause::run, py::call_guard<py::gil_scoped_release>())
-      .def("__repr__", [](const SingleGPUExecutionPause &) { return std::string("<SingleGPUExeuction>"); });
+      .def("__repr__", [](const SingleGPUExecutionPause &) { return std::string("<SingleGPUExecution: one device>"); });
 
   m.def("SingleGPURunBatch", &SingleGPURunBatch);
 
@@ -223,9 +223,9 @@ PYBIND11_MODULE(corelib, m) {
       .def("run", &CLS::run, py::call_guard<py::gil_scoped_release>())                      \
       .def("numDevice", &CLS::numDevice)                                                    \
       .def("__repr__", [](const CLS &e) { return "<" #CLS ": " + std::to_string(e.numDevice()) + ">"; });
-  BIND_MULTI(MultiGPUExecutionNaive, "A naive way to utilize multiple GPUs (one GPU in this build)")
-  BIND_MULTI(MultiGPUExecutionDyn, "multi GPU execution with dynamic load balancing (one GPU in this build)")
-  BIND_MULTI(MultiGPUExecutionDynPause, "multi GPU execution with dynamic load balancing (one GPU in this build)")
+  BIND_MULTI(MultiGPUExecutionNaive, "Static split of roots over GPUs (one GPU here)")
+  BIND_MULTI(MultiGPUExecutionDyn, "Roots handed out to GPUs on demand (one GPU here)")
+  BIND_MULTI(MultiGPUExecutionDynPause, "Roots handed out to GPUs on demand (one GPU here)")
 #undef BIND_MULTI
 
   m.def("allCount", &allCount);

@@ -89,7 +89,16 @@ class GraphDataManagerImpl : public GraphDataManager {
       hg.Euid = (const int32_t *)h.Euid_h->data();
       tm_graph *g = nullptr;
       tm_throw(tm_graph_upload(device, &hg, &g), "tm_graph_upload");
-      it = resident_.emplace(key, std::shared_ptr<tm_graph>(g, [](tm_graph *p) { tm_graph_free(p); })).first;
+      std::shared_ptr<tm_graph> sg(g, [](tm_graph *p) { tm_graph_free(p); });
+      // labels travel with the graph, as the reference's moveAsync copies them
+      const size_t ne = h.edgeFeatures ? h.edgeFeatures->size() / sizeof(int32_t) : 0;
+      const size_t nv = h.verticesFeatures ? h.verticesFeatures->size() / sizeof(int32_t) : 0;
+      if ((ne == h.numEdges && ne > 0) || nv > 0)
+        tm_throw(tm_graph_set_features(g, ne == h.numEdges ? (const int32_t *)h.edgeFeatures->data() : nullptr,
+                                       ne == h.numEdges ? (int32_t)ne : 0,
+                                       nv ? (const int32_t *)h.verticesFeatures->data() : nullptr, (int32_t)nv),
+                 "tm_graph_set_features");
+      it = resident_.emplace(key, std::move(sg)).first;
     }
     s = describe(device, &h);
     s.graph = it->second;
@@ -102,6 +111,14 @@ std::unique_ptr<GraphDataManager> MakeGraphDataManager(int totalDevice) {
   return std::unique_ptr<GraphDataManager>(new GraphDataManagerImpl(totalDevice));
 }
 
+// A device copy owned by a shared pointer that frees it on the graph's device
+// (the graph handle is held by the deleter, so it outlives the copy).
+static std::shared_ptr<void> device_copy(const std::shared_ptr<tm_graph> &g, const void *host, size_t bytes) {
+  void *p = nullptr;
+  tm_throw(tm_device_copy(g.get(), host, (int64_t)bytes, &p), "tm_device_copy");
+  return std::shared_ptr<void>(p, [g](void *q) { tm_device_free(g.get(), q); });
+}
+
 DeviceJobData::DeviceJobData(const DeviceGraphData &deviceGraph, const HostMotifData &hostMotif) {
   device = deviceGraph.device;
   graphNumEdges = deviceGraph.numEdges;
@@ -111,6 +128,23 @@ DeviceJobData::DeviceJobData(const DeviceGraphData &deviceGraph, const HostMotif
   motifNumVertices = hostMotif.numVertices;
   motif = hostMotif.edges;
   minfol_ = hostMotif.minfo;
+  if (!graph) return;  // not resident yet (alloc without moveAsync): no device view
+  tm_device_arrays a;
+  tm_throw(tm_graph_arrays(graph.get(), &a), "tm_graph_arrays");
+  Eg_d = (const TemporalEdge *)a.Eg;
+  inEdgesV_d = a.inEdgesV;
+  outEdgesV_d = a.outEdgesV;
+  inEdgesR_d = a.inEdgesR;
+  outEdgesR_d = a.outEdgesR;
+  edgefeatures_d = a.edgeFeatures;
+  nodefeatures_d = a.verticesFeatures;
+  Euid_d = a.Euid;
+  for (auto &mi : minfol_) {
+    mi.arrR = mi.io == 1 ? outEdgesR_d : inEdgesR_d;
+    mi.arrV = mi.io == 1 ? outEdgesV_d : inEdgesV_d;
+  }
+  Em_d_ = device_copy(graph, motif.data(), sizeof(TemporalEdge) * motif.size());
+  minfo_ = device_copy(graph, minfol_.data(), sizeof(MotifEdgeInfoV1) * minfol_.size());
 }
 
 std::vector<int32_t> DeviceJobData::motifSrc() const {

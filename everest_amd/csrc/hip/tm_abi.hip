@@ -19,6 +19,7 @@
 #include "tm_abi.h"
 #include "tm_kernel.cuh"
 #include "tm_internal.h"
+#include "tm_variants.h"
 
 namespace {
 
@@ -192,7 +193,7 @@ __global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops
     st_dev(&s->waiters, 0);
     // fresh ring tags for this launch (the host clears the ring before 2^18)
     __hip_atomic_fetch_add(&s->gen, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s->t0 = (long long)wall_clock64();
+    __hip_atomic_store(&s->t0, (long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (i < 16) st_dev(&s->stats[i], 0ull);
   if (i < n) { st_dev(&counts[i], 0ull); st_dev(&stops[i], 0); }
@@ -319,52 +320,14 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
   return TM_OK;
 }
 
-template <int MAXN, bool ENUM, bool INSTR, int WPE, bool MULTI>
-static tm_status occupancy(int lds, int *blocks) {
-  HIPTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks, tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE, MULTI>, 64, lds));
-  return TM_OK;
-}
-
-template <int MAXN, bool ENUM, bool INSTR, int WPE, bool MULTI>
-static tm_status launch_t(Slot &sl, const tm_kernel_args &a, int grid, int lds) {
-  hipLaunchKernelGGL((tmk::mine_kernel<TM_MAXE, MAXN, ENUM, INSTR, tmk::DynSpec, WPE, MULTI>), dim3(grid), dim3(64),
-                     lds, sl.stream, a);
-  HIPTRY(hipGetLastError());
-  return TM_OK;
-}
-
-// Kernel variants: node map of 4 or 8, count / enumerate / instrumented
-// count; counting also comes with a register cap of 80 VGPRs (wpe 6) for
-// launches whose LDS stack leaves room for more than 20 wavefronts per CU,
-// and in a single-query and a multi-query (batch) form.
+// Kernel variants (tm_variants.h): node map of 4 or 8, count / enumerate /
+// instrumented count; counting also comes with register caps of 96 and 80
+// VGPRs, and in a single-query and a multi-query (batch) form.
 // OCC: report the occupancy in *blocks instead of launching.
-template <bool OCC, int MAXN, bool ENUM, bool INSTR, int WPE, bool MULTI>
-static tm_status variant(Slot &sl, int lds, int *blocks, const tm_kernel_args *a, int grid) {
-  if (OCC) return occupancy<MAXN, ENUM, INSTR, WPE, MULTI>(lds, blocks);
-  return launch_t<MAXN, ENUM, INSTR, WPE, MULTI>(sl, *a, grid, lds);
-}
-
-template <bool OCC, int MAXN>
-static tm_status dispatch_n(Slot &sl, bool enumerate, bool instr, int wpe, bool multi, int lds, int *blocks,
-                            const tm_kernel_args *a, int grid) {
-  if (instr) return variant<OCC, MAXN, false, true, 1, false>(sl, lds, blocks, a, grid);
-  if (enumerate) return variant<OCC, MAXN, true, false, 1, false>(sl, lds, blocks, a, grid);
-  if (multi) {
-    if (wpe == 6) return variant<OCC, MAXN, false, false, 6, true>(sl, lds, blocks, a, grid);
-    if (wpe == 5) return variant<OCC, MAXN, false, false, 5, true>(sl, lds, blocks, a, grid);
-    return variant<OCC, MAXN, false, false, 1, true>(sl, lds, blocks, a, grid);
-  }
-  if (wpe == 6) return variant<OCC, MAXN, false, false, 6, false>(sl, lds, blocks, a, grid);
-  if (wpe == 5) return variant<OCC, MAXN, false, false, 5, false>(sl, lds, blocks, a, grid);
-  return variant<OCC, MAXN, false, false, 1, false>(sl, lds, blocks, a, grid);
-}
-
 template <bool OCC>
 static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wpe, bool multi, int lds, int *blocks,
                           const tm_kernel_args *a, int grid) {
-  return nmap == 4 ? dispatch_n<OCC, 4>(sl, enumerate, instr, wpe, multi, lds, blocks, a, grid)
-                   : dispatch_n<OCC, 8>(sl, enumerate, instr, wpe, multi, lds, blocks, a, grid);
+  return tmv::pick(nmap, enumerate, instr, wpe, multi)(OCC, sl.stream, lds, blocks, a, grid);
 }
 
 // Plan and launch the mining kernel for staged queries [q0, q0 + nq) of
@@ -665,8 +628,8 @@ tm_status collect(tm_graph *g, Slot &sl, const tm_sched &hs) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) sl.last_kernel_ms = ms;
   g->last_slot = (int)(&sl - g->slot);
+  if (hs.error & ~4) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(hs.error) + ")");
   if (hs.error & 4) return fail(TM_ERR_TIMEOUT, "query exceeded its time limit; counts discarded");
-  if (hs.error) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(hs.error) + ")");
   return TM_OK;
 }
 
@@ -889,12 +852,79 @@ tm_status tm_graph_free(tm_graph *g) {
   for (Slot &sl : g->slot)
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
   for (void *p : g->allocs) (void)hipFree(p);
+  if (g->edge_features) (void)hipFree(g->edge_features);
+  if (g->vertex_features) (void)hipFree(g->vertex_features);
   for (Slot &sl : g->slot) {
     if (sl.ev0) (void)hipEventDestroy(sl.ev0);
     if (sl.ev1) (void)hipEventDestroy(sl.ev1);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
   delete g;
+  return TM_OK;
+}
+
+tm_status tm_graph_arrays(const tm_graph *g, tm_device_arrays *out) {
+  if (!g || !out) return fail(TM_ERR_ARG, "null argument");
+  out->num_edges = g->E;
+  out->num_vertices = g->V;
+  out->Eg = g->view.Eg;
+  out->inEdgesV = g->view.inPos;
+  out->outEdgesV = g->view.outPos;
+  out->inEdgesR = g->view.inR;
+  out->outEdgesR = g->view.outR;
+  out->Euid = g->view.Euid;
+  out->edgeFeatures = g->edge_features;
+  out->verticesFeatures = g->vertex_features;
+  out->num_vertex_features = g->num_vertex_features;
+  return TM_OK;
+}
+
+tm_status tm_graph_set_features(tm_graph *g, const int32_t *edge_features, int32_t num_edge_features,
+                                const int32_t *vertex_features, int32_t num_vertex_features) {
+  if (!g || num_edge_features < 0 || num_vertex_features < 0 || (num_edge_features && !edge_features) ||
+      (num_vertex_features && !vertex_features))
+    return fail(TM_ERR_ARG, "bad argument");
+  if (num_edge_features && num_edge_features != g->E) return fail(TM_ERR_ARG, "one edge label per edge");
+  std::lock_guard<std::mutex> lk(g->mtx);
+  HIPTRY(hipSetDevice(g->device));
+  auto replace = [&](int **dst, int *n, const int32_t *src, int32_t cnt) -> tm_status {
+    if (*dst) {
+      HIPTRY(hipFree(*dst));
+      *dst = nullptr;
+      *n = 0;
+    }
+    if (cnt == 0) return TM_OK;
+    HIPTRY(hipMalloc((void **)dst, sizeof(int32_t) * (size_t)cnt));
+    HIPTRY(hipMemcpy(*dst, src, sizeof(int32_t) * (size_t)cnt, hipMemcpyHostToDevice));
+    *n = cnt;
+    return TM_OK;
+  };
+  tm_status st = replace(&g->edge_features, &g->num_edge_features, edge_features, num_edge_features);
+  if (st == TM_OK) st = replace(&g->vertex_features, &g->num_vertex_features, vertex_features, num_vertex_features);
+  return st;
+}
+
+tm_status tm_device_copy(tm_graph *g, const void *host, int64_t bytes, void **device_ptr) {
+  if (!g || !device_ptr || bytes < 0 || (bytes > 0 && !host)) return fail(TM_ERR_ARG, "bad argument");
+  HIPTRY(hipSetDevice(g->device));
+  void *p = nullptr;
+  HIPTRY(hipMalloc(&p, (size_t)std::max<int64_t>(bytes, 4)));
+  if (bytes > 0) {
+    hipError_t e = hipMemcpy(p, host, (size_t)bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return fail(TM_ERR_HIP, std::string("tm_device_copy: ") + hipGetErrorString(e));
+    }
+  }
+  *device_ptr = p;
+  return TM_OK;
+}
+
+tm_status tm_device_free(tm_graph *g, void *device_ptr) {
+  if (!g) return fail(TM_ERR_ARG, "null graph");
+  if (!device_ptr) return TM_OK;
+  HIPTRY(hipSetDevice(g->device));
+  HIPTRY(hipFree(device_ptr));
   return TM_OK;
 }
 

@@ -67,6 +67,9 @@ struct tm_graph {
   tm_tuning class_tune[TM_NCLASSES] = {};  // per shape class (autotuner), used when a call brings none
   bool class_set[TM_NCLASSES] = {};
   std::mutex mtx;            // one API call at a time per graph handle
+  int *edge_features = nullptr;    // labels (tm_graph_set_features), not read by the kernels
+  int *vertex_features = nullptr;
+  int num_edge_features = 0, num_vertex_features = 0;
 };
 
 namespace tmh {

@@ -137,6 +137,9 @@ __device__ __forceinline__ unsigned ld_rlx(const unsigned *p) {
 __device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ long long ld_rlx(const long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ void st_rlx(unsigned long long *p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -432,12 +435,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 
   for (;;) {
     if ((++iter & 63u) == 0u) {  // deadline check (wall clock, uniform)
-      if (args.limit_ticks > 0 && lane == 0 && (long long)wall_clock64() - sched->t0 > args.limit_ticks)
+      if (args.limit_ticks > 0 && lane == 0 && (long long)wall_clock64() - ld_rlx(&sched->t0) > args.limit_ticks) {
+        atomicOr(&sched->error, 4);
         atomicOr(&sched->abort, 1);
-      if (uni(ld_rlx(&sched->abort))) {
-        if (lane == 0) atomicOr(&sched->error, 4);
-        break;
       }
+      // abort: the deadline passed (error bit 2) or a protocol wait gave up (bit 3)
+      if (uni(ld_rlx(&sched->abort))) break;
       // enumeration cap reached: the subtrees in flight are dropped (their
       // rows could not be written), so the launch ends as soon as every
       // wavefront sees the flag instead of after finishing them
@@ -491,7 +494,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           if (lane == 0) {
             b = ld_rlx(&sched->busy);
             ab = ld_rlx(&sched->abort);
-            if (!ab && args.limit_ticks > 0 && (long long)wall_clock64() - sched->t0 > args.limit_ticks) {
+            if (!ab && args.limit_ticks > 0 && (long long)wall_clock64() - ld_rlx(&sched->t0) > args.limit_ticks) {
               atomicOr(&sched->abort, 1);
               atomicOr(&sched->error, 4);
               ab = 1;
@@ -963,11 +966,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
               if (ld_rlx(&sched->abort) || ++w > (1u << 24)) { ok = 0; break; }
               __builtin_amdgcn_s_sleep(8);
             }
-            if (!ok) atomicOr(&sched->error, 8);
+            if (!ok) {
+              // record t will never be written, so its ticket holder could
+              // only leave through the abort flag: raise it, and every
+              // wavefront (waiting ones included) leaves; the host reports
+              // the error bit as a failed launch
+              atomicOr(&sched->error, 8);
+              atomicOr(&sched->abort, 1);
+            }
           }
         }
         t = (unsigned)uni((int)__shfl((int)t, 0));
-        if (!uni(__shfl(ok, 0))) break;  // abort, or the error above ends the launch
+        if (!uni(__shfl(ok, 0))) break;  // abort: leave the donation loop; the main loop sees the flag
         const int cur0 = f_cur[b], end0 = f_end[b];
         const int mid = cur0 + (end0 - cur0) / 2;
         const unsigned tag = pool_tag(gen, t >> args.pool_shift);

@@ -80,6 +80,7 @@ struct Arena {
     }
     return false;
   }
+  bool empty() const { return size == 0 || (free_.size() == 1 && free_.begin()->second == size); }
   void release(size_t off, size_t n) {
     n = (n + 255) & ~size_t(255);
     auto it = free_.emplace(off, n).first;
@@ -153,7 +154,16 @@ bool launch_next(tm_server *s, int k) {
     if (r.cap >= 0) {
       size_t off = 0;
       const size_t len = sizeof(int32_t) * (size_t)std::max<int64_t>(r.cap, 1) * r.M;
-      if (!s->arena.alloc(len, &off)) continue;
+      if (!s->arena.alloc(len, &off)) {
+        if (!s->arena.empty()) continue;  // waits for running enumerations to free the arena
+        // cannot fit even an empty arena (submit rejects such requests): fail it
+        r.status = set_error(TM_ERR_NOMEM, "enumeration rows exceed the server's arena");
+        r.finished = true;
+        r.done = Clock::now();
+        s->waiting.erase(it);
+        s->cv_done.notify_all();
+        return true;
+      }
       r.arena_off = off;
       r.arena_len = len;
       pick.push_back(*it);
@@ -318,7 +328,9 @@ tm_status tm_server_create(tm_graph *g, const tm_server_config *cfg, tm_server *
     if (cfg->arena_bytes < 0) return set_error(TM_ERR_ARG, "arena_bytes must be >= 0");
     if (cfg->slots) s->slots = cfg->slots;
     if (cfg->max_batch) s->max_batch = cfg->max_batch;
-    s->arena_bytes = cfg->arena_bytes ? (size_t)cfg->arena_bytes : (size_t)1 << 30;
+    // the arena hands out 256-byte-aligned pieces: its size is rounded down to match
+    s->arena_bytes = cfg->arena_bytes ? ((size_t)cfg->arena_bytes & ~(size_t)255) : (size_t)1 << 30;
+    if (s->arena_bytes == 0) return set_error(TM_ERR_ARG, "arena_bytes must be 0 (default) or at least 256");
     s->tune = cfg->tune;
     const tm_tuning zero{};
     s->own_tune = memcmp(&cfg->tune, &zero, sizeof(zero)) != 0;
@@ -381,8 +393,15 @@ tm_status tm_server_submit(tm_server *s, int32_t num_edges, const int32_t *msrc,
   r->delta = delta;
   r->cap = enum_cap;
   r->cls = depth_class(md.M, md.n);
-  if (enum_cap >= 0 && sizeof(int32_t) * (size_t)std::max<int64_t>(enum_cap, 1) * num_edges > s->arena_bytes)
-    return set_error(TM_ERR_NOMEM, "enumeration rows exceed the server's arena");
+  if (enum_cap >= 0) {
+    // the piece the arena would hand out (rounded up to 256 bytes) must fit
+    // an empty arena, and its size must not overflow
+    const int64_t most = (int64_t)(s->arena_bytes / (sizeof(int32_t) * (size_t)num_edges));
+    if (enum_cap > most) return set_error(TM_ERR_NOMEM, "enumeration rows exceed the server's arena");
+    const size_t len = sizeof(int32_t) * (size_t)std::max<int64_t>(enum_cap, 1) * num_edges;
+    if (((len + 255) & ~(size_t)255) > s->arena_bytes)
+      return set_error(TM_ERR_NOMEM, "enumeration rows exceed the server's arena");
+  }
   r->submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

@@ -5,9 +5,9 @@
 // reference's corelib compiles against this header.  Differences:
 //   * no Boost / fmt / range-v3: mapped files use POSIX mmap, printing uses
 //     std::ostream;
-//   * device memory is reached through the C ABI in tm_abi.h: DeviceGraphData
-//     and DeviceJobData hold a resident tm_graph handle instead of one raw
-//     pointer per array;
+//   * device memory is allocated through the C ABI in tm_abi.h: DeviceGraphData
+//     holds a resident tm_graph handle, and DeviceJobData exposes that graph's
+//     device arrays under the reference's field names;
 //   * GraphDataManager keeps uploaded graphs resident per device and hands the
 //     same device copy to every execution of the same host graph;
 //   * loader errors throw std::runtime_error instead of exiting.
@@ -45,8 +45,9 @@ struct TemporalEdge {
 
 bool operator==(const TemporalEdge &lhs, const TemporalEdge &rhs);
 
-// include/data.h:33-40 (arrR/arrV are bound to host CSR pointers here; the
-// device kernels read the motif through tm_abi.h instead)
+// include/data.h:33-40.  HostMotifData leaves arrR/arrV null; DeviceJobData
+// binds them to the device out-CSR (io == 1) or in-CSR (otherwise), as
+// src/core/gpu/data.cu:515-523 does.
 struct MotifEdgeInfoV1 {
   int baseNode, constraintNode;
   int mappedNodes;
@@ -273,17 +274,27 @@ struct HostMotifData {
 
 std::ostream &operator<<(std::ostream &o, const HostMotifData &obj);
 
-// include/data.h:304-320: the graph on a device plus one motif.
+// include/data.h:304-320: the graph on a device plus one motif.  The device
+// pointers are the resident graph's arrays (valid while `graph` lives); Em_d()
+// and minfo() are device copies of the motif edges and of minfol_, whose
+// arrR/arrV point at the device CSR (src/core/gpu/data.cu:496-529), so a
+// worker kernel written against the reference's data.h reads the same fields.
 struct DeviceJobData {
   int device{0};
   size_t graphNumEdges{0}, graphNumVertices{0};
+  const TemporalEdge *Eg_d{nullptr};
+  const int *inEdgesV_d{nullptr}, *outEdgesV_d{nullptr}, *inEdgesR_d{nullptr}, *outEdgesR_d{nullptr};
+  const int *edgefeatures_d{nullptr}, *nodefeatures_d{nullptr};
+  const int *Euid_d{nullptr};
   std::shared_ptr<tm_graph> graph;
-  size_t motifNumEdges{0}, motifNumVertices{0};
-  std::vector<TemporalEdge> motif;
-  std::vector<MotifEdgeInfoV1> minfol_;
 
-  const TemporalEdge *Em_d() const { return motif.data(); }
-  const MotifEdgeInfoV1 *minfo() const { return minfol_.data(); }
+  size_t motifNumEdges{0}, motifNumVertices{0};
+  std::shared_ptr<void> Em_d_, minfo_;  // device copies (freed with the last holder)
+  std::vector<TemporalEdge> motif;      // host copy of the motif edges
+  std::vector<MotifEdgeInfoV1> minfol_; // host copy of minfo(), device CSR pointers bound
+
+  const TemporalEdge *Em_d() const { return (const TemporalEdge *)Em_d_.get(); }
+  const MotifEdgeInfoV1 *minfo() const { return (const MotifEdgeInfoV1 *)minfo_.get(); }
   std::vector<int32_t> motifSrc() const;
   std::vector<int32_t> motifDst() const;
 

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define TM_ABI_VERSION 3
+#define TM_ABI_VERSION 4
 #define TM_MAX_MOTIF_EDGES 8
 #define TM_MAX_MOTIF_NODES 8
 
@@ -99,6 +99,47 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
 
 /* Release a graph's device memory (reference: DevicePtrDeleter, data.cu:364-370). */
 tm_status tm_graph_free(tm_graph *g);
+
+/*
+ * Device pointers of a resident graph's reference arrays, the fields of
+ * DeviceGraphData / DeviceJobData a hand-written worker reads
+ * (include/data.h:195-209 and 304-320; bound in
+ * src/core/gpu/data.cu:496-529): Eg_d (TemporalEdge {u, v, t} per edge),
+ * in/outEdgesV_d, in/outEdgesR_d, Euid_d, edgefeatures_d and
+ * nodefeatures_d (null until tm_graph_set_features).  Valid until
+ * tm_graph_free.
+ */
+typedef struct tm_device_arrays {
+  int32_t num_edges;
+  int32_t num_vertices;
+  const void *Eg;
+  const int32_t *inEdgesV;
+  const int32_t *outEdgesV;
+  const int32_t *inEdgesR;
+  const int32_t *outEdgesR;
+  const int32_t *Euid;
+  const int32_t *edgeFeatures;     /* num_edges labels, or null */
+  const int32_t *verticesFeatures; /* num_vertex_features labels, or null */
+  int32_t num_vertex_features;
+} tm_device_arrays;
+tm_status tm_graph_arrays(const tm_graph *g, tm_device_arrays *out);
+
+/*
+ * Copy edge and vertex labels (HostGraphData::edgeFeatures and
+ * verticesFeatures, include/data.h:174-186) next to a resident graph, as
+ * GraphDataManager::moveAsync does (src/core/gpu/data.cu:440-472).  Replaces
+ * labels set before.  The counting and enumeration kernels do not read them.
+ */
+tm_status tm_graph_set_features(tm_graph *g, const int32_t *edge_features, int32_t num_edge_features,
+                                const int32_t *vertex_features, int32_t num_vertex_features);
+
+/*
+ * A device copy of `bytes` bytes of host memory on the graph's device (the
+ * motif edges and MotifEdgeInfoV1 records of a DeviceJobData,
+ * src/core/gpu/data.cu:510-528), and its release.
+ */
+tm_status tm_device_copy(tm_graph *g, const void *host, int64_t bytes, void **device_ptr);
+tm_status tm_device_free(tm_graph *g, void *device_ptr);
 
 /*
  * Work counters of the graph's last launch: [0] wavefront steps, [1] candidate

@@ -10,7 +10,7 @@ Follows:
 
 Pinned by: the compiled reference reader in ``oracle/_ref`` (built from the
 reference Graph.cc by ``oracle/Makefile``) on every reference motif fixture and
-on random motif files (tests/test_oracle_motif.py).
+on random motif files (tests/test_oracle.py::test_motif_reader_matches_reference_build).
 """
 from dataclasses import dataclass
 from typing import List, Tuple

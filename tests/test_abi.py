@@ -25,10 +25,16 @@ def test_every_declared_symbol_is_exported():
 
 def test_version_and_no_device_ok():
     L = abi.lib()
-    assert L.tm_abi_version() == 3
+    text = open(os.path.join(ROOT, "include", "tm_abi.h")).read()
+    assert L.tm_abi_version() == int(re.search(r"#define TM_ABI_VERSION (\d+)", text).group(1))
     assert L.tm_device_count() >= 0
 
 
 def test_plugin_exports_getworker():
     L = ctypes.CDLL(os.path.join(abi.LIB_DIR, "libTM.so"), mode=os.RTLD_GLOBAL if hasattr(os, "RTLD_GLOBAL") else 0)
+    assert hasattr(L, "getWorker")
+
+
+def test_handwritten_plugin_exports_getworker():
+    L = ctypes.CDLL(os.path.join(abi.LIB_DIR, "libCSR.so"), mode=os.RTLD_GLOBAL if hasattr(os, "RTLD_GLOBAL") else 0)
     assert hasattr(L, "getWorker")

@@ -109,3 +109,33 @@ def test_stock_plugin_and_executions(inputs):
     div.setWorker(w2)
     assert div.run(3600) == want
     del ex, div, w, w2
+
+
+def test_handwritten_worker_reads_device_job_data(inputs):
+    """A worker written against the reference's DeviceJobData fields (Eg_d,
+    minfo()[i].arrR / arrV on the device: libCSR.so, one thread per root,
+    plain CSR scans) loads through GPUWorkerLibManager and counts
+    oracle-exact through SingleGPUExecution, cumulatively like the
+    reference's workers."""
+    from everest_amd import core
+    from everest_amd.abi import LIB_DIR
+    from oracle import motif as omotif
+    root, g = inputs
+    el = core.data.EdgeListLoader(str(root / "graphs" / "zipf.txt"))
+    hg = core.data.GraphDataLoader(el, core.data.FeatureLoader(el, "")).createGraphData()
+    dm = core.data.MakeGraphDataManager(1)
+    lm = core.GPUWorkerLibManager()
+    lm.openLib(os.path.join(LIB_DIR, "libCSR.so"))
+    for pat, d in (("cycle3.txt", 3600), ("M1.txt", 1800), ("M3.txt", 3600), ("M4.txt", 7200), ("hub5.txt", 3600)):
+        motif_path = os.path.join(str(root), "motifs", pat)
+        hm = core.data.HostMotifData(motif_path)
+        w = lm.construct("CSR", 0)
+        ex = core.SingleGPUExecution(dm)
+        ex.setHostGraph(hg)
+        ex.setHostMotif(hm)
+        ex.setWorker(w)
+        say("hand-written worker", pat, d)
+        want = oracle.count(g, omotif.read_motif(motif_path), d)
+        assert ex.run(d) == want, pat
+        assert ex.run(d) == 2 * want, pat  # cumulative
+        del ex, w

@@ -106,7 +106,7 @@ def test_reference_work_counters(small_cases):
         assert got_c == want_c and got_w == want_w, (i, got_w, want_w)
 
 
-def test_pl8m_root_subranges_and_timeouts():
+def test_pl8m_root_subranges():
     """Bit-exact on the 8M-edge power-law graph over seeded root subranges
     (counts are additive over root ranges), all 36 3-edge motifs."""
     abi = _abi()
