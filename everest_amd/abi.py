@@ -38,7 +38,7 @@ class Tuning(ctypes.Structure):
                 ("steal_every", ctypes.c_int32), ("steal_max", ctypes.c_int32),
                 ("steal_min", ctypes.c_int32), ("steal_after_us", ctypes.c_int32),
                 ("waves_per_simd_hint", ctypes.c_int32), ("sb_flush", ctypes.c_int32),
-                ("sb_inline", ctypes.c_int32)]
+                ("sb_inline", ctypes.c_int32), ("tail_swap", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -75,6 +75,7 @@ def lib():
         L.tm_last_stats.argtypes = [P, P, I32]
         L.tm_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.tm_count_work.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P, P]
+        L.tm_count_bytes.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P, P]
         L.tm_shape_class.argtypes = [I32, P, P, ctypes.POINTER(I32)]
         L.tm_set_class_tuning.argtypes = [P, I32, ctypes.POINTER(Tuning)]
         L.tm_batch_prepare.argtypes = [P, I32, P, P, P, P, ctypes.POINTER(Tuning), ctypes.POINTER(P)]
@@ -184,6 +185,15 @@ class DeviceGraph:
         check(lib().tm_count_work(self.handle, len(edges), _p(src), _p(dst), int(delta), int(beg), int(end),
                                   ctypes.byref(t), _p(out), _p(work)))
         return int(out[0]), [int(x) for x in work]
+
+    def count_bytes(self, edges, delta, beg=0, end=-1, tune=None):
+        """(count, bytes read by this library's own search), from the metered kernel."""
+        src, dst = motif_arrays(edges)
+        out = np.zeros(2, dtype=np.uint64)
+        t = tune if tune is not None else Tuning()
+        check(lib().tm_count_bytes(self.handle, len(edges), _p(src), _p(dst), int(delta), int(beg), int(end),
+                                   ctypes.byref(t), _p(out[:1]), _p(out[1:])))
+        return int(out[0]), int(out[1])
 
     def count(self, edges, delta, beg=0, end=-1, tune=None):
         src, dst = motif_arrays(edges)

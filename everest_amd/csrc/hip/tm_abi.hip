@@ -235,6 +235,18 @@ namespace tmh {
 
 tm_status set_error(tm_status s, const std::string &msg) { return fail(s, msg); }
 
+static int shape_class_of(int M, const int32_t *src, const int32_t *dst) {
+  int n = 0;
+  for (int k = 0; k < M; k++) n = std::max(n, 1 + std::max(src[k], dst[k]));
+  return std::max(1, M - 2) * 2 + (n <= 4 ? 0 : 1);
+}
+
+// The tuning table entry of a motif's shape class, if one is installed.
+const tm_tuning *class_tuning_of(tm_graph *g, int M, const int32_t *src, const int32_t *dst) {
+  const int cls = shape_class_of(M, src, dst);
+  return (cls >= 0 && cls < TM_NCLASSES && g->class_set[cls]) ? &g->class_tune[cls] : nullptr;
+}
+
 tm_status dev_alloc(tm_graph *g, void **p, size_t bytes) {
   if (bytes == 0) bytes = 4;
   hipError_t e = hipMalloc(p, bytes);
@@ -325,7 +337,7 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
 // VGPRs, and in a single-query and a multi-query (batch) form.
 // OCC: report the occupancy in *blocks instead of launching.
 template <bool OCC>
-static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wpe, bool multi, int lds, int *blocks,
+static tm_status dispatch(Slot &sl, int nmap, bool enumerate, int instr, int wpe, bool multi, int lds, int *blocks,
                           const tm_kernel_args *a, int grid) {
   return tmv::pick(nmap, enumerate, instr, wpe, multi)(OCC, sl.stream, lds, blocks, a, grid);
 }
@@ -334,7 +346,7 @@ static tm_status dispatch(Slot &sl, int nmap, bool enumerate, bool instr, int wp
 // scratch.h_queries (nq <= TM_LAUNCH_MAXQ): size the persistent grid from
 // occupancy, pick the chunk size over the union of the root ranges, upload,
 // reset, launch.
-tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr, bool multi,
+tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, int instr, bool multi,
                tm_launch_fn fn, void *user, Plan *pl) {
   if (nq > 1 && (enumerate || instr || fn || !multi)) return fail(TM_ERR_INTERNAL, "several queries need the batch kernel");
   Scratch &s = sl.s;
@@ -356,7 +368,7 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   // the survivors with which a step expands its own when nothing is buffered
   int sb_flush = 0, sb_inline = 0;
   bool sb_auto = false;  // the library's choice: on unless its buffer costs resident wavefronts
-  if (!enumerate && !instr) {
+  if (!enumerate && instr != 1) {  // the counting and the metered kernels
     sb_flush = TM_SB_FLUSH_DEFAULT;
     sb_inline = TM_SB_INLINE_DEFAULT;
     sb_auto = !(tune && tune->sb_flush != 0);
@@ -367,7 +379,9 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   // ints), the survivor buffer (counting), then the launch's per-query counts
   auto lds_of = [&](int flush) {
     const int sb = flush > 0 ? (2 + nmap) * tmk::WAVE : 0;
-    return (frames * per_frame + 2 * tmk::TM_ROOTS + sb) * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
+    const int t2 = (enumerate || instr == 1) ? 0 : 2 * tmk::WAVE;  // two-edge tail side table (counting)
+    return (frames * per_frame + 2 * tmk::TM_ROOTS + sb + t2) * (int)sizeof(int) +
+           nq * (int)sizeof(unsigned long long);
   };
   int wpe = 0;  // register cap of the counting kernel: 1 (none), 5 (96 VGPRs) or 6 (80 VGPRs)
   if (tune && tune->waves_per_simd_hint > 0)
@@ -415,7 +429,7 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
     qt[2] = qd.root_end;
     // start rule of edge 1 and, outside the instrumented kernel (which
     // follows the reference's search tree), the root's second rule
-    qt[3] = qd.m.tmode[1] | (instr ? 0 : qd.m.xmask[0] << TM_QT_XRULE_SHIFT);
+    qt[3] = qd.m.tmode[1] | (instr == 1 ? 0 : qd.m.xmask[0] << TM_QT_XRULE_SHIFT);
     s.h_prefix[q0 + i] = total_blocks;
     if (qd.root_end > qd.root_beg) {
       const long long nb = (qd.root_end - (qd.root_beg & ~3) + 255) / 256;
@@ -471,6 +485,8 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
     const long long us = (tune && tune->steal_after_us > 0) ? tune->steal_after_us : 1000;
     a.steal_after_ticks = us * g->wall_rate_khz / 1000;
   }
+  // two-edge tail: frames of edge M-2 with at least this many window candidates
+  a.t2_min = (tune && tune->tail_swap < 0) ? 0x7fffffff : ((tune && tune->tail_swap > 0) ? tune->tail_swap : 32);
   sl.last_claim = claim_blocks;
   sl.last_grid = grid;
   sl.last_lds = lds;
@@ -531,7 +547,7 @@ tm_status enqueue(tm_graph *g, Slot &sl, const Plan &pl, bool timing, bool count
   return TM_OK;
 }
 
-tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
+tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, int instr,
                  bool multi, tm_launch_fn fn, void *user) {
   Plan pl;
   tm_status st = plan(g, sl, q0, nq, enumerate, tune, instr, multi, fn, user, &pl);
@@ -642,6 +658,117 @@ tm_status finish(tm_graph *g, Slot &sl, int q0, int nq, uint64_t *counts) {
   HIPTRY(hipMemcpyAsync(counts, s.d_counts + q0, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, sl.stream));
   HIPTRY(hipStreamSynchronize(sl.stream));
   return collect(g, sl, hs);
+}
+
+// Counts of one motif over consecutive root ranges [cuts[i], cuts[i + 1])
+// (i < n <= TM_LAUNCH_MAXQ), in one launch of the batch kernel.
+static tm_status count_ranges(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta,
+                              const int32_t *cuts, int n, const tm_tuning *tune, uint64_t *counts) {
+  std::vector<int32_t> ne(n, M), dl(n, delta), bg(cuts, cuts + n), en(cuts + 1, cuts + n + 1);
+  // query i: the same motif (stride 0), roots [cuts[i], cuts[i + 1])
+  tm_status st = stage_queries(g, sl, 0, n, nullptr, ne.data(), src, dst, 0, dl.data(), bg.data(), en.data());
+  if (st == TM_OK) st = launch(g, sl, 0, n, false, tune, false, true);
+  if (st == TM_OK) st = finish(g, sl, 0, n, counts);
+  return st;
+}
+
+// Every row of the roots [b, e) (fewer than `room` of them) into host rows.
+static tm_status enum_all(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta,
+                          int32_t b, int32_t e, int64_t room, std::vector<int32_t> *rows) {
+  int32_t *d_rows = nullptr;
+  HIPTRY(hipMalloc(&d_rows, sizeof(int32_t) * (size_t)std::max<int64_t>(room, 1) * M));
+  tm_status st = stage_queries(g, sl, 0, 1, nullptr, &M, src, dst, M, &delta, &b, &e);
+  uint64_t c = 0;
+  if (st == TM_OK) {
+    sl.s.h_queries[0].rows = d_rows;
+    sl.s.h_queries[0].cap = room;
+    st = launch(g, sl, 0, 1, true, nullptr, false, false);
+  }
+  if (st == TM_OK) st = finish(g, sl, 0, 1, &c);
+  if (st == TM_OK && (int64_t)c >= room) st = fail(TM_ERR_INTERNAL, "enumeration found more rows than counted");
+  if (st == TM_OK) {
+    rows->resize((size_t)c * M);
+    if (c) {
+      hipError_t he = hipMemcpy(rows->data(), d_rows, sizeof(int32_t) * rows->size(), hipMemcpyDeviceToHost);
+      if (he != hipSuccess) st = fail(TM_ERR_HIP, hipGetErrorString(he));
+    }
+  }
+  (void)hipFree(d_rows);
+  return st;
+}
+
+tm_status enum_cut(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta, int32_t beg,
+                   int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows, uint64_t *count,
+                   int64_t *rows_written) {
+  (void)tune;  // the cut is found with the counting kernel's own launch parameters
+  *rows_written = 0;
+  if (cap == 0) { *count = 0; return TM_OK; }
+  const tm_tuning *ct = class_tuning_of(g, M, src, dst);
+  // Find the root r* at which the cumulative count over [beg, r] first
+  // reaches the cap: ranges of growing width after beg, then the range that
+  // holds the crossing cut into TM_CUT_PARTS parts, down to one root.
+  const int P = 64;
+  uint64_t before = 0;       // matches of the roots [beg, lo)
+  int32_t lo = beg;
+  long long width = 1 << 14;
+  int32_t rstar = -1;
+  uint64_t c_star = 0;       // matches of root r*
+  std::vector<int32_t> cuts(P + 1);
+  std::vector<uint64_t> cnt(P);
+  tm_status st;
+  for (;;) {
+    const int32_t hi = (int32_t)std::min<long long>(end, (long long)lo + width);
+    if (hi <= lo) return fail(TM_ERR_INTERNAL, "the cap was reached but the counts never reach it");
+    const long long span = hi - lo;
+    int n = (int)std::min<long long>(P, span);
+    for (int i = 0; i <= n; i++) cuts[i] = lo + (int32_t)(span * i / n);
+    if ((st = count_ranges(g, sl, M, src, dst, delta, cuts.data(), n, ct, cnt.data())) != TM_OK) return st;
+    int k = -1;
+    uint64_t acc = before;
+    for (int i = 0; i < n; i++) {
+      if (acc + cnt[i] >= (uint64_t)cap) { k = i; break; }
+      acc += cnt[i];
+    }
+    if (k < 0) {  // not within [lo, hi): move on, twice as wide
+      before = acc;
+      lo = hi;
+      width *= 2;
+      continue;
+    }
+    before = acc;
+    lo = cuts[k];
+    if (cuts[k + 1] - cuts[k] == 1) {
+      rstar = cuts[k];
+      c_star = cnt[k];
+      break;
+    }
+    width = cuts[k + 1] - cuts[k];  // refine inside the part that holds the crossing
+  }
+  // rows of [beg, r*): all of them (fewer than cap); rows of r*: the
+  // k = cap - before smallest by their Euid tuples
+  std::vector<int32_t> head, tail;
+  if ((st = enum_all(g, sl, M, src, dst, delta, beg, rstar, (int64_t)before + 1, &head)) != TM_OK) return st;
+  if ((int64_t)head.size() != (int64_t)before * M) return fail(TM_ERR_INTERNAL, "enumeration and count disagree");
+  const int64_t kMaxRootRows = (int64_t)1 << 27;
+  if ((int64_t)c_star > kMaxRootRows)
+    return fail(TM_ERR_NOMEM, "the root at the cap holds more than 2^27 matches; its rows cannot be ordered");
+  if ((st = enum_all(g, sl, M, src, dst, delta, rstar, rstar + 1, (int64_t)c_star + 1, &tail)) != TM_OK) return st;
+  if ((int64_t)tail.size() != (int64_t)c_star * M) return fail(TM_ERR_INTERNAL, "enumeration and count disagree");
+  const int64_t k = cap - (int64_t)before;
+  std::vector<int64_t> idx((size_t)c_star);
+  for (int64_t i = 0; i < (int64_t)c_star; i++) idx[(size_t)i] = i;
+  auto less = [&](int64_t a, int64_t b) {
+    return std::lexicographical_compare(tail.begin() + a * M, tail.begin() + (a + 1) * M, tail.begin() + b * M,
+                                        tail.begin() + (b + 1) * M);
+  };
+  if (k < (int64_t)c_star) std::nth_element(idx.begin(), idx.begin() + k, idx.end(), less);
+  std::sort(idx.begin(), idx.begin() + k, less);
+  memcpy(rows, head.data(), sizeof(int32_t) * head.size());
+  for (int64_t i = 0; i < k; i++)
+    memcpy(rows + (size_t)(before + i) * M, tail.data() + (size_t)idx[(size_t)i] * M, sizeof(int32_t) * M);
+  *rows_written = cap;
+  *count = before + c_star;
+  return TM_OK;
 }
 
 }  // namespace tmh
@@ -1017,6 +1144,22 @@ tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, con
   return TM_OK;
 }
 
+tm_status tm_count_bytes(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
+                         int32_t beg, int32_t end, const tm_tuning *tune, uint64_t *count, uint64_t *bytes) {
+  if (!g || !msrc || !mdst || !count || !bytes) return fail(TM_ERR_ARG, "null argument");
+  if (num_edges >= 2 && num_edges <= TM_MAX_MOTIF_EDGES)
+    tune = class_tuning(g, motif_class(num_edges, msrc, mdst), tune);
+  std::lock_guard<std::mutex> lk(g->mtx);
+  HIPTRY(hipSetDevice(g->device));
+  Slot &sl = g->slot[0];
+  tm_status st = stage_queries(g, sl, 0, 1, nullptr, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
+  if (st != TM_OK) return st;
+  if ((st = launch(g, sl, 0, 1, false, tune, 2, false)) != TM_OK) return st;
+  if ((st = finish(g, sl, 0, 1, count)) != TM_OK) return st;
+  *bytes = sl.last_stats[TM_STAT_REF_X1];
+  return TM_OK;
+}
+
 // Launch shape class of a count query: frame-stack levels and node-map width
 // (they set the LDS a wavefront needs, hence the occupancy of its launch).
 static int shape_class(int M, int n) { return std::max(1, M - 2) * 2 + (n <= 4 ? 0 : 1); }
@@ -1108,6 +1251,7 @@ static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, 
   tm_status st = stage_queries(g, sl, 0, 1, nullptr, &num_edges, msrc, mdst, num_edges, &delta, &beg, &end);
   if (st != TM_OK) return st;
   Scratch &s = sl.s;
+  const int32_t rbeg = s.h_queries[0].root_beg, rend = s.h_queries[0].root_end;
   int32_t *d_rows = nullptr;
   const size_t row_bytes = sizeof(int32_t) * (size_t)std::max<int64_t>(cap, 1) * num_edges;
   HIPTRY(hipMalloc(&d_rows, row_bytes));
@@ -1115,15 +1259,22 @@ static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, 
   s.h_queries[0].cap = cap;
   if ((st = launch(g, sl, 0, 1, true, tune, false, false, fn, user)) != TM_OK) { (void)hipFree(d_rows); return st; }
   st = finish(g, sl, 0, 1, count);
-  if (st == TM_OK) {
-    int64_t n = std::min<int64_t>((int64_t)*count, cap);
-    if (n > 0) {
-      hipError_t he = hipMemcpy(rows, d_rows, sizeof(int32_t) * (size_t)n * num_edges, hipMemcpyDeviceToHost);
+  if (st == TM_OK && *count < (uint64_t)cap) {
+    // below the cap no root was held back: every match is written
+    if (*count > 0) {
+      hipError_t he = hipMemcpy(rows, d_rows, sizeof(int32_t) * (size_t)*count * num_edges, hipMemcpyDeviceToHost);
       if (he != hipSuccess) st = fail(TM_ERR_HIP, hipGetErrorString(he));
     }
-    if (rows_written) *rows_written = n;
+    if (rows_written) *rows_written = (int64_t)*count;
+  } else if (st == TM_OK) {
+    // at the cap: replace the schedule-dependent subset by the deterministic one
+    (void)hipFree(d_rows);
+    d_rows = nullptr;
+    int64_t nw = 0;
+    st = enum_cut(g, sl, num_edges, msrc, mdst, delta, rbeg, rend, cap, tune, rows, count, &nw);
+    if (rows_written) *rows_written = st == TM_OK ? nw : 0;
   }
-  (void)hipFree(d_rows);
+  if (d_rows) (void)hipFree(d_rows);
   return st;
 }
 

@@ -87,20 +87,21 @@ struct Plan {
   unsigned long long *d_counts = nullptr;  // the launch's per-query counts
   int *d_stops = nullptr;                  // and stop flags
   int grid = 0, lds = 0, nmap = 4, wpe = 1, q0 = 0, nq = 0;
-  bool enumerate = false, instr = false, multi = false;
+  bool enumerate = false, multi = false;
+  int instr = 0;  // 1: reference work, 2: metered bytes
   tm_launch_fn fn = nullptr;
   void *user = nullptr;
 };
 // Plan a launch of staged queries [q0, q0 + nq) of the slot's staging
 // (occupancy, variant, claim size, query table) and upload its tables.
-tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr, bool multi,
+tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, int instr, bool multi,
                tm_launch_fn fn, void *user, Plan *pl);
 // Enqueue a planned launch on the slot's stream: ring clear when the
 // generation wraps (host side), reset kernel, mining kernel (between the
 // slot's timing events when timing), statistics kernel.
 tm_status enqueue(tm_graph *g, Slot &sl, const Plan &pl, bool timing, bool count);
 // plan + enqueue (multi: the batch kernel; fn: a caller's kernel).
-tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, bool instr,
+tm_status launch(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_tuning *tune, int instr,
                  bool multi, tm_launch_fn fn = nullptr, void *user = nullptr);
 // Count n launches on the slot (each advances the device generation by one),
 // clearing the ring first on `stream` if the generation would wrap.
@@ -109,5 +110,15 @@ tm_status count_launch(tm_graph *g, Slot &sl, hipStream_t stream, int n);
 tm_status collect(tm_graph *g, Slot &sl, const tm_sched &hs);
 // Synchronous: wait for the slot's launch and copy out its counts.
 tm_status finish(tm_graph *g, Slot &sl, int q0, int nq, uint64_t *counts);
+// The tuning table entry of a motif's shape class (nullptr when none is set).
+const tm_tuning *class_tuning_of(tm_graph *g, int M, const int32_t *src, const int32_t *dst);
+// Deterministic rows of an enumeration that reaches its cap (cap > 0 rows of
+// M ints into host `rows`): every row of the roots before r*, the root at
+// which the cumulative count over [beg, r*] first reaches the cap, and the
+// remaining rows from root r*'s rows in increasing lexicographic order of
+// their Euid tuples.  *count = the matches of the roots [beg, r*].
+tm_status enum_cut(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta, int32_t beg,
+                   int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows, uint64_t *count,
+                   int64_t *rows_written);
 
 }  // namespace tmh

@@ -32,17 +32,26 @@ constexpr int TM_ROOTS = WAVE + 1;  // window table entries: one per lane's root
 
 // ------------------------------------------------------------------ helpers
 
-__device__ __forceinline__ int lower_bound_i(const int *__restrict__ a, int lo, int hi, int x) {
+// Bytes a lane's search reads from memory, kept only by the metering variant
+// of the counting kernel (the product kernel passes no meter, and every
+// `if (mt)` folds away): 4 per int32 word, 12 per edge record.
+struct Meter {
+  unsigned long long b;
+};
+
+__device__ __forceinline__ int lower_bound_i(const int *__restrict__ a, int lo, int hi, int x, Meter *mt = nullptr) {
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
+    if (mt) mt->b += 4;
     if (a[mid] < x) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
 
-__device__ __forceinline__ int upper_bound_i(const int *__restrict__ a, int lo, int hi, int x) {
+__device__ __forceinline__ int upper_bound_i(const int *__restrict__ a, int lo, int hi, int x, Meter *mt = nullptr) {
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
+    if (mt) mt->b += 4;
     if (a[mid] <= x) lo = mid + 1; else hi = mid;
   }
   return lo;
@@ -184,36 +193,41 @@ __device__ __forceinline__ int pick(const int (&t)[N], int j) {
 }
 
 // first position whose time is > tl (the window end as a position bound)
-__device__ __forceinline__ int pend_of(const tm_graph_view &g, long long tl) {
+__device__ __forceinline__ int pend_of(const tm_graph_view &g, long long tl, Meter *mt = nullptr) {
   long long k = (tl - (long long)g.tmin) >> g.bshift;
   if (k < 0) return 0;
   if (k >= g.nbuckets) return g.E;
   int lo = g.bucket[k], hi = g.bucket[k + 1];
-  return upper_bound_i(g.T, lo, hi, (int)tl);
+  if (mt) mt->b += 8;
+  return upper_bound_i(g.T, lo, hi, (int)tl, mt);
 }
 
 // first index in [lo, hi) with a[i] >= x, probing outward from lo
 // (exponential search: cheap when the answer is near lo, log otherwise)
-__device__ __forceinline__ int gallop_lb(const int *__restrict__ a, int lo, int hi, int x) {
+__device__ __forceinline__ int gallop_lb(const int *__restrict__ a, int lo, int hi, int x, Meter *mt = nullptr) {
+  if (mt && lo < hi) mt->b += 4;
   if (lo >= hi || a[lo] >= x) return lo;
   int prev = lo, step = 1;
   for (;;) {
     int nxt = prev + step;
-    if (nxt >= hi) return lower_bound_i(a, prev + 1, hi, x);
-    if (a[nxt] >= x) return lower_bound_i(a, prev + 1, nxt, x);
+    if (nxt >= hi) return lower_bound_i(a, prev + 1, hi, x, mt);
+    if (mt) mt->b += 4;
+    if (a[nxt] >= x) return lower_bound_i(a, prev + 1, nxt, x, mt);
     prev = nxt;
     step <<= 1;
   }
 }
 
 // first index in [lo, hi) with a[i] > x, probing outward from lo
-__device__ __forceinline__ int gallop_ub(const int *__restrict__ a, int lo, int hi, int x) {
+__device__ __forceinline__ int gallop_ub(const int *__restrict__ a, int lo, int hi, int x, Meter *mt = nullptr) {
+  if (mt && lo < hi) mt->b += 4;
   if (lo >= hi || a[lo] > x) return lo;
   int prev = lo, step = 1;
   for (;;) {
     int nxt = prev + step;
-    if (nxt >= hi) return upper_bound_i(a, prev + 1, hi, x);
-    if (a[nxt] > x) return upper_bound_i(a, prev + 1, nxt, x);
+    if (nxt >= hi) return upper_bound_i(a, prev + 1, hi, x, mt);
+    if (mt) mt->b += 4;
+    if (a[nxt] > x) return upper_bound_i(a, prev + 1, nxt, x, mt);
     prev = nxt;
     step <<= 1;
   }
@@ -222,22 +236,25 @@ __device__ __forceinline__ int gallop_ub(const int *__restrict__ a, int lo, int 
 // Entries of the pair segment key inside one node's (neighbour, position)-
 // sorted list [lo, hi) whose positions lie in (p, pend).
 __device__ __forceinline__ int pair_count(const int *__restrict__ pnbr, const int *__restrict__ ppos,
-                                          int lo, int hi, int key, int p, int pend) {
-  int s0 = lower_bound_i(pnbr, lo, hi, key);
+                                          int lo, int hi, int key, int p, int pend, Meter *mt = nullptr) {
+  int s0 = lower_bound_i(pnbr, lo, hi, key, mt);
+  if (mt && s0 < hi) mt->b += 4;
   if (s0 >= hi || pnbr[s0] != key) return 0;
-  int s1 = gallop_ub(pnbr, s0 + 1, hi, key);
-  int c0 = upper_bound_i(ppos, s0, s1, p);
-  int c1 = gallop_lb(ppos, c0, s1, pend);
+  int s1 = gallop_ub(pnbr, s0 + 1, hi, key, mt);
+  int c0 = upper_bound_i(ppos, s0, s1, p, mt);
+  int c1 = gallop_lb(ppos, c0, s1, pend, mt);
   return c1 - c0;
 }
 
 // Edges x -> y with positions in (p, pend), searched in the shorter of out(x)
 // and in(y).
-__device__ __forceinline__ int edge_count(const tm_graph_view &g, int x, int y, int p, int pend) {
+__device__ __forceinline__ int edge_count(const tm_graph_view &g, int x, int y, int p, int pend,
+                                          Meter *mt = nullptr) {
   int olo = g.outR[x], ohi = g.outR[x + 1];
   int ilo = g.inR[y], ihi = g.inR[y + 1];
-  if (ohi - olo <= ihi - ilo) return pair_count(g.outPairNbr, g.outPairPos, olo, ohi, y, p, pend);
-  return pair_count(g.inPairNbr, g.inPairPos, ilo, ihi, x, p, pend);
+  if (mt) mt->b += 16;
+  if (ohi - olo <= ihi - ilo) return pair_count(g.outPairNbr, g.outPairPos, olo, ohi, y, p, pend, mt);
+  return pair_count(g.inPairNbr, g.inPairPos, ilo, ihi, x, p, pend, mt);
 }
 
 // Candidate list of motif edge j: slots [lo, hi) of `pos` (and `nbr` for new
@@ -251,7 +268,7 @@ struct CandList {
 
 template <class SP, int MAXN>
 __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_motif_dev &md, int j,
-                                               const int (&m)[MAXN], int p) {
+                                               const int (&m)[MAXN], int p, Meter *mt = nullptr) {
   CandList L;
   L.dir = 0;
   const int kind = SP::kind(md, j);
@@ -261,17 +278,19 @@ __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_
     L.pos = g.outPos;
     L.nbr = g.outNbr;
     L.hi = g.outR[x + 1];
+    if (mt) mt->b += 8;
     if (mode == TM_S_OUT_SELF) L.lo = g.outSelf[p] + 1;
     else if (mode == TM_S_OUT_NEXT) L.lo = g.outNext[p];
-    else L.lo = upper_bound_i(g.outPos, g.outR[x], L.hi, p);
+    else L.lo = upper_bound_i(g.outPos, g.outR[x], L.hi, p, mt);
   } else if (kind == TM_K_NEW_SRC) {
     const int y = sel(m, SP::b(md, j));
     L.pos = g.inPos;
     L.nbr = g.inNbr;
     L.hi = g.inR[y + 1];
+    if (mt) mt->b += 8;
     if (mode == TM_S_IN_SELF) L.lo = g.inSelf[p] + 1;
     else if (mode == TM_S_IN_NEXT) L.lo = g.inNext[p];
-    else L.lo = upper_bound_i(g.inPos, g.inR[y], L.hi, p);
+    else L.lo = upper_bound_i(g.inPos, g.inR[y], L.hi, p, mt);
   } else {
     const int x = sel(m, SP::a(md, j)), y = sel(m, SP::b(md, j));
     L.nbr = nullptr;
@@ -279,22 +298,24 @@ __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_
       // the same pair as edge j-1: continue right after p inside its segment
       L.pos = g.outPairPos;
       L.lo = g.outPairSelf[p] + 1;
-      L.hi = gallop_ub(g.outPairNbr, L.lo, g.outR[x + 1], y);
+      if (mt) mt->b += 8;
+      L.hi = gallop_ub(g.outPairNbr, L.lo, g.outR[x + 1], y, mt);
     } else {
       int olo = g.outR[x], ohi = g.outR[x + 1];
       int ilo = g.inR[y], ihi = g.inR[y + 1];
+      if (mt) mt->b += 16;
       int s0, s1;
       if (ohi - olo <= ihi - ilo) {
-        s0 = lower_bound_i(g.outPairNbr, olo, ohi, y);
-        s1 = gallop_ub(g.outPairNbr, s0, ohi, y);
+        s0 = lower_bound_i(g.outPairNbr, olo, ohi, y, mt);
+        s1 = gallop_ub(g.outPairNbr, s0, ohi, y, mt);
         L.pos = g.outPairPos;
       } else {
-        s0 = lower_bound_i(g.inPairNbr, ilo, ihi, x);
-        s1 = gallop_ub(g.inPairNbr, s0, ihi, x);
+        s0 = lower_bound_i(g.inPairNbr, ilo, ihi, x, mt);
+        s1 = gallop_ub(g.inPairNbr, s0, ihi, x, mt);
         L.pos = g.inPairPos;
         L.dir = 1;
       }
-      L.lo = upper_bound_i(L.pos, s0, s1, p);
+      L.lo = upper_bound_i(L.pos, s0, s1, p, mt);
       L.hi = s1;
     }
   }
@@ -306,13 +327,14 @@ __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_
 template <class SP, int MAXN>
 __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g, const tm_motif_dev &md,
                                                          int j, const int (&m)[MAXN], const CandList &L,
-                                                         int p, int pend, int scan_max) {
-  const int b1 = gallop_lb(L.pos, L.lo, L.hi, pend);
+                                                         int p, int pend, int scan_max, Meter *mt = nullptr) {
+  const int b1 = gallop_lb(L.pos, L.lo, L.hi, pend, mt);
   int c = b1 - L.lo;
   if (c <= 0 || SP::kind(md, j) == TM_K_CLOSE) return c > 0 ? (unsigned long long)c : 0ull;
   const int nm = SP::mapped(md, j);
   if (c <= scan_max) {
     int bad = 0;
+    if (mt) mt->b += 4 * (unsigned long long)c;
     for (int s = L.lo; s < b1; s++) bad += in_first(m, nm, L.nbr[s]) ? 1 : 0;
     return (unsigned long long)(c - bad);
   }
@@ -321,14 +343,125 @@ __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g,
   const int base = out ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j));
 #pragma unroll
   for (int i = 0; i < MAXN; i++)
-    if (i < nm) c -= out ? edge_count(g, base, m[i], p, pend) : edge_count(g, m[i], base, p, pend);
+    if (i < nm) c -= out ? edge_count(g, base, m[i], p, pend, mt) : edge_count(g, m[i], base, p, pend, mt);
   return (unsigned long long)c;
+}
+
+// ------------------------------------------------ two-edge tail (count mode)
+//
+// For a frame of the second-to-last motif edge j = M-2 (its candidate list
+// A, bounded by the window), the matches of the last two edges are
+//   sum over candidates c of A of (matches of edge k = M-1 after c).
+// When edge k does not touch the node edge j introduces, its candidate list
+// B is the same for every c, so the sum can be taken the other way round:
+//   sum over candidates q of B of (candidates c of A before q that are
+//   valid: new node not mapped, and different from q's new node).
+// The kernel takes that way when B's window part is the shorter list (hub
+// nodes: thousands of candidates for edge j against a handful for edge k),
+// and deals B's candidates to the lanes instead of A's.
+
+// lvl word of a frame: level | A's pair direction << 8 | swapped << 9 |
+// B's pair direction << 10 | root index << 16
+constexpr int LV_DIR = 1 << 8;
+constexpr int LV_SWAP = 1 << 9;
+constexpr int LV_BDIR = 1 << 10;
+
+template <class SP>
+__device__ __forceinline__ bool tail2_ok(const tm_motif_dev &md) {
+  const int M = SP::M(md);
+  if (M < 3) return false;
+  if (SP::kind(md, M - 2) == TM_K_CLOSE) return true;
+  const int w = SP::mapped(md, M - 2);  // the node edge M-2 introduces
+  return SP::a(md, M - 1) != w && SP::b(md, M - 1) != w;
+}
+
+// Edge k's candidates (k = M-1) with positions in (pa0, pend), pa0 the
+// position of A's first candidate: [*lo, *hi) in the list of *bdir.
+template <class SP, int MAXN>
+__device__ __forceinline__ void tail2_range(const tm_graph_view &g, const tm_motif_dev &md, const int (&m)[MAXN],
+                                            int pa0, int pend, int *lo, int *hi, int *bdir, Meter *mt = nullptr) {
+  const int k = SP::M(md) - 1;
+  const int kind = SP::kind(md, k);
+  *bdir = 0;
+  if (kind == TM_K_NEW_DST) {
+    const int x = sel(m, SP::a(md, k));
+    const int e = g.outR[x + 1];
+    if (mt) mt->b += 8;
+    *lo = upper_bound_i(g.outPos, g.outR[x], e, pa0, mt);
+    *hi = gallop_lb(g.outPos, *lo, e, pend, mt);
+  } else if (kind == TM_K_NEW_SRC) {
+    const int y = sel(m, SP::b(md, k));
+    const int e = g.inR[y + 1];
+    if (mt) mt->b += 8;
+    *lo = upper_bound_i(g.inPos, g.inR[y], e, pa0, mt);
+    *hi = gallop_lb(g.inPos, *lo, e, pend, mt);
+  } else {
+    const int x = sel(m, SP::a(md, k)), y = sel(m, SP::b(md, k));
+    const int olo = g.outR[x], ohi = g.outR[x + 1];
+    const int ilo = g.inR[y], ihi = g.inR[y + 1];
+    const int *pos;
+    int s0, s1;
+    if (mt) mt->b += 16;
+    if (ohi - olo <= ihi - ilo) {
+      s0 = lower_bound_i(g.outPairNbr, olo, ohi, y, mt);
+      s1 = gallop_ub(g.outPairNbr, s0, ohi, y, mt);
+      pos = g.outPairPos;
+    } else {
+      s0 = lower_bound_i(g.inPairNbr, ilo, ihi, x, mt);
+      s1 = gallop_ub(g.inPairNbr, s0, ihi, x, mt);
+      pos = g.inPairPos;
+      *bdir = 1;
+    }
+    *lo = upper_bound_i(pos, s0, s1, pa0, mt);
+    *hi = gallop_lb(pos, *lo, s1, pend, mt);
+  }
+}
+
+// Matches of the last two edges whose last edge is edge k's candidate at
+// `slot` (list direction bdir): edge j's candidates in A [alo, ahi) (pair
+// direction adir) before it, less those whose new node is mapped or equal to
+// the candidate's new node.
+template <class SP, int MAXN>
+__device__ __forceinline__ unsigned long long tail2_item(const tm_graph_view &g, const tm_motif_dev &md,
+                                                         const int (&m)[MAXN], int alo, int ahi, int adir,
+                                                         int slot, int bdir, Meter *mt = nullptr) {
+  const int M = SP::M(md);
+  const int j = M - 2, k = M - 1;
+  const int nm = SP::mapped(md, j);
+  const int kk = SP::kind(md, k);
+  int qpos, z = -1;
+  if (kk == TM_K_NEW_DST) {
+    qpos = g.outPos[slot];
+    z = g.outNbr[slot];
+  } else if (kk == TM_K_NEW_SRC) {
+    qpos = g.inPos[slot];
+    z = g.inNbr[slot];
+  } else {
+    qpos = bdir ? g.inPairPos[slot] : g.outPairPos[slot];
+  }
+  if (mt) mt->b += kk == TM_K_CLOSE ? 4 : 8;
+  if (kk != TM_K_CLOSE && in_first(m, nm, z)) return 0ull;
+  const int kj = SP::kind(md, j);
+  const int *apos = kj == TM_K_NEW_DST ? g.outPos : (kj == TM_K_NEW_SRC ? g.inPos : (adir ? g.inPairPos : g.outPairPos));
+  int r = lower_bound_i(apos, alo, ahi, qpos, mt) - alo;
+  if (r <= 0 || kj == TM_K_CLOSE) return r > 0 ? (unsigned long long)r : 0ull;
+  // edge j's candidates before q whose new node is taken: edges between
+  // A's node and that node in [first candidate, q)
+  const int p0 = apos[alo] - 1;
+  if (mt) mt->b += 4;
+  const bool out = kj == TM_K_NEW_DST;
+  const int base = out ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j));
+#pragma unroll
+  for (int i = 0; i < MAXN; i++)
+    if (i < nm) r -= out ? edge_count(g, base, m[i], p0, qpos, mt) : edge_count(g, m[i], base, p0, qpos, mt);
+  if (kk != TM_K_CLOSE) r -= out ? edge_count(g, base, z, p0, qpos, mt) : edge_count(g, z, base, p0, qpos, mt);
+  return (unsigned long long)r;
 }
 
 // ------------------------------------------------------------------ kernel
 
 // LDS frame stack, structure of arrays, capacity FR frames:
-//   cur, end, p (last matched position), pend, lvl (level | dir << 8),
+//   cur, end, p (last matched position), pend, lvl (level | dir << 8 | swap flags),
 //   m[MAXN][FR] mapping, and in enumeration mode e[MAXE][FR] matched positions.
 // Reference work of one resolved candidate list (the instrumented variant):
 // the reference scans list entries after p until the first one outside the
@@ -358,7 +491,10 @@ __device__ __forceinline__ unsigned long long ref_probes(const tm_graph_view &g,
 // wavefronts per CU when LDS allows).
 // MULTI: the launch carries several queries (batches); a single-query launch
 // skips the search for the query of a root block.
-template <int MAXE, int MAXN, bool ENUM, bool INSTR, class SP = DynSpec, int WPE = 1, bool MULTI = false>
+// INSTR: 0 = the counting kernel; 1 = instrumented with the reference's work
+// (walks the reference's search tree: no pruning, no two-edge tail); 2 =
+// metered: the counting kernel's own search, adding up the bytes it reads.
+template <int MAXE, int MAXN, bool ENUM, int INSTR, class SP = DynSpec, int WPE = 1, bool MULTI = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void mine_kernel(tm_kernel_args args) {
   extern __shared__ int lds[];
   const int lane = threadIdx.x;
@@ -378,12 +514,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   // survivor buffer (counting, when args.sb_flush > 0): candidates of one
   // level that passed the injectivity and pruning tests, waiting to be
   // expanded up to 64 at a time
-  const bool COMPACT = !ENUM && !INSTR && args.sb_flush > 0;
+  const bool COMPACT = !ENUM && INSTR != 1 && args.sb_flush > 0;
   int *sb_pos = r_tl + TM_ROOTS;
   int *sb_ridx = sb_pos + (COMPACT ? WAVE : 0);
   int *sb_m = sb_ridx + (COMPACT ? WAVE : 0);
+  // two-edge tail (counting): edge M-2's candidate range [sw_lo, sw_hi) of a
+  // swapped frame, at the frame's stack index mod 64 (the frames of the
+  // deepest level are pushed in one step, at most 64 of them)
+  constexpr bool T2 = !ENUM && INSTR != 1;
+  int *sw_lo = sb_m + (COMPACT ? MAXN * WAVE : 0);
+  int *sw_hi = sw_lo + (T2 ? WAVE : 0);
   // per-query counts of this wavefront, flushed once at the end
-  unsigned long long *q_cnt = (unsigned long long *)(sb_m + (COMPACT ? MAXN * WAVE : 0));
+  unsigned long long *q_cnt = (unsigned long long *)(sw_hi + (T2 ? WAVE : 0));
   for (int q = lane; q < args.nq; q += WAVE) q_cnt[q] = 0;
   __syncthreads();
 
@@ -396,7 +538,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   unsigned st_steps = 0, st_roots = 0, st_push = 0;  // uniform, per wavefront
   unsigned long long st_items = 0;
   unsigned st_leaves = 0;  // per lane
-  unsigned long long rx1 = 0, rx2 = 0, rp = 0;  // reference work (INSTR only)
+  unsigned long long rx1 = 0, rx2 = 0, rp = 0;  // reference work (INSTR == 1 only)
+  Meter meter{0};                                // bytes read (INSTR == 2 only)
+  Meter *const mt = INSTR == 2 ? &meter : nullptr;
   unsigned st_don = 0, st_take = 0, st_capexit = 0;
   unsigned long long st_work = 0;            // wall-clock ticks this wavefront held work
   long long work_t0 = (long long)wall_clock64();
@@ -605,7 +749,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           st_roots += (unsigned)(min(max(cend - blk, 0), TM_BLOCK) - min(max(cbeg - blk, 0), TM_BLOCK));
           const unsigned mn = min(min(x.x, x.y), min(x.z, x.w));
           const bool near = cdelta >= 0 && mn <= (unsigned)cdelta;
-          if (!INSTR && __ballot(near) == 0ull) { blk = -1; continue; }
+          if (mt) meter.b += 16 * (1 + __popc((unsigned)cxmask));  // the gap words of this lane's 4 roots
+          if (INSTR != 1 && __ballot(near) == 0ull) { blk = -1; continue; }
           quarter = -1;
         }
         if (++quarter == 4) { blk = -1; continue; }
@@ -613,7 +758,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         const unsigned lg = pick4(bgap, quarter);
         inq = r >= cbeg && r < cend;
         pass = inq && cdelta >= 0 && lg <= (unsigned)cdelta;
-        if (__ballot(pass) != 0ull || (INSTR && __ballot(inq) != 0ull)) qsel = cq;
+        if (__ballot(pass) != 0ull || (INSTR == 1 && __ballot(inq) != 0ull)) qsel = cq;
       }
       if (qsel < 0) continue;  // every root chunk is handed out
       if (qsel != qcur) {
@@ -630,7 +775,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       const int r = blk + 4 * lane + quarter;
       int u = 0, v = 0, t = 0;
       bool gvalid = false;
-      if (pass || (INSTR && inq)) {
+      if (pass || (INSTR == 1 && inq)) {
+        if (mt) meter.b += 12;
         const tm_edge e = g.Eg[r];
         u = e.u; v = e.v; t = e.t;
         gvalid = u != v;
@@ -644,11 +790,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       const long long tl = (long long)t + q.delta;
       const int tli = tl > 0x7fffffffll ? 0x7fffffff : (int)tl;
       if (ok) {
-        L = cand_start<SP>(g, md, 1, m, r);
+        L = cand_start<SP>(g, md, 1, m, r, mt);
+        if (mt && L.lo < L.hi) meter.b += 8;
         ok = L.lo < L.hi && (long long)g.T[L.pos[L.lo]] <= tl;
       }
       int pend = 0;
-      if constexpr (INSTR) {
+      if constexpr (INSTR == 1) {
         if (gvalid) {
           const int pe = pend_of(g, tl);
           (SP::kind(md, 1) == TM_K_CLOSE ? rx2 : rx1) += 1;
@@ -656,20 +803,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
       }
       if (ok) {
-        pend = pend_of(g, tl);
-        if (!INSTR) ok = (pend - r - 1) >= (M - 1);
+        pend = pend_of(g, tl, mt);
+        if (INSTR != 1) ok = (pend - r - 1) >= (M - 1);
       }
       const int last = ENUM ? M - 1 : M - 2;  // last motif edge handled per candidate
       if (last == 0) {
         // count mode, M == 2: edge 1 is the final edge
-        if (ok) { cnt += leaf_count<SP>(g, md, 1, m, L, r, pend, args.scan_max); st_leaves++; }
+        if (ok) { cnt += leaf_count<SP>(g, md, 1, m, L, r, pend, args.scan_max, mt); st_leaves++; }
         continue;
       }
       int cb = 0, ce = 0, dir = L.dir;
       if (ok) {
         cb = L.lo;
-        ce = gallop_lb(L.pos, L.lo, L.hi, pend);
+        ce = gallop_lb(L.pos, L.lo, L.hi, pend, mt);
         ok = ce > cb;
+      }
+      // two-edge tail of a 3-edge motif: edge 1 is the second-to-last edge
+      int swp = 0, t2lo = 0, t2hi = 0;
+      if (T2 && M == 3 && ok && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
+        int blo, bhi, bdir;
+        if (mt) meter.b += 4;
+        tail2_range<SP>(g, md, m, L.pos[cb], pend, &blo, &bhi, &bdir, mt);
+        if (bhi - blo < ce - cb) {
+          swp = LV_SWAP | (bdir ? LV_BDIR : 0);
+          t2lo = cb; t2hi = ce;
+          cb = blo; ce = bhi;
+          ok = ce > cb;
+        }
       }
       unsigned long long bm = __ballot(ok);
       int n = __popcll(bm);
@@ -678,7 +838,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         r_pend[lane] = pend;  // the window of this lane's root (no frame of an older root is left)
         r_tl[lane] = tli;
         int w = top + __popcll(bm & lt_mask);
-        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = 0 | (dir << 8) | (lane << 16);
+        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = 0 | (dir << 8) | swp | (lane << 16);
+        if (T2 && swp) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
 #pragma unroll
         for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
         if constexpr (ENUM) f_e[w] = r;
@@ -735,7 +896,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const int myf = top - 1 - rank;
     st_steps++;
     st_items += total;
-    int slot = 0, pend = 0, dir = 0, tl = 0, ridx = 0;
+    int slot = 0, pend = 0, dir = 0, tl = 0, ridx = 0, lvw = 0;
     int m[MAXN];
     int e[ENUM ? MAXE : 1];
 #pragma unroll
@@ -743,7 +904,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     if (has) {
       slot = f_cur[myf] + off;
       const int lv = f_lvl[myf];
-      dir = (lv >> 8) & 0xff;
+      lvw = lv;
+      dir = (lv >> 8) & 1;
       ridx = (lv >> 16) & 0xff;
       pend = r_pend[ridx];
       tl = r_tl[ridx];
@@ -762,9 +924,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const int j = L + 1;  // motif edge of the candidates
     const int kind = SP::kind(md, j);
     const int tmn = (j + 1 < M) ? SP::tmode(md, j + 1) : -1;  // pruning rule of edge j + 1
+    // a swapped frame (two-edge tail): the slot is a candidate of the last
+    // edge; its matches with the earlier candidates of edge j are counted here
+    const bool swp = T2 && has && (lvw & LV_SWAP);
+    if (T2 && swp) {
+      cnt += tail2_item<SP>(g, md, m, sw_lo[myf & (WAVE - 1)], sw_hi[myf & (WAVE - 1)], dir, slot,
+                            (lvw & LV_BDIR) ? 1 : 0, mt);
+      st_leaves++;
+    }
     bool valid = false;
     int pos = 0, tfn = 0;
-    if (has) {
+    if (has && !swp) {
       int node = 0;
       // position, neighbour and the next edge's first-candidate time: three
       // loads at the same slot of one list, issued together
@@ -774,7 +944,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       // further rules: each later edge (k >= j + 2) at a node of this edge
       // needs that node's next edge after this position in the window (not
       // in the instrumented kernel, which follows the reference's search tree)
-      const int xm = INSTR ? 0 : SP::xmask(md, j);
+      const int xm = INSTR == 1 ? 0 : SP::xmask(md, j);
+      // position, neighbour (new node), the next edge's time and the further rules' times
+      if (mt) meter.b += 4 * (1 + (kind != TM_K_CLOSE ? 1 : 0) + (tmn >= 0 ? 1 : 0) + __popc((unsigned)xm));
       int tx = -0x7fffffff - 1;
 #pragma unroll
       for (int rr = 0; rr < TM_NTFIRST; rr++)
@@ -819,7 +991,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       }
     }
     if (!buffer) {
-    if constexpr (INSTR) {
+    if constexpr (INSTR == 1) {
       if (valid && j + 1 < M) {
         (SP::kind(md, j + 1) == TM_K_CLOSE ? rx2 : rx1) += 1;
         rp += ref_probes<SP>(g, md, j + 1, m, pos, pend);
@@ -840,8 +1012,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
       } else {
         if (valid && (tmn < 0 || tfn <= tl)) {
-          const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
-          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max);
+          const CandList LL = cand_start<SP>(g, md, j + 1, m, pos, mt);
+          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, mt);
           st_leaves++;
         }
       }
@@ -850,20 +1022,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       continue;
     }
     int cb = 0, ce = 0, cdir = 0;
+    int swc = 0, t2lo = 0, t2hi = 0;
     if (valid && tmn >= 0) valid = tfn <= tl;
     if (valid) {
-      const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
+      const CandList LL = cand_start<SP>(g, md, j + 1, m, pos, mt);
       cb = LL.lo;
       cdir = LL.dir;
-      ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend);
+      ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend, mt);
       valid = ce > cb;
+      if (T2 && valid && j == M - 3 && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
+        int blo, bhi, bdir;
+        if (mt) meter.b += 4;
+        tail2_range<SP>(g, md, m, LL.pos[cb], pend, &blo, &bhi, &bdir, mt);
+        if (bhi - blo < ce - cb) {
+          swc = LV_SWAP | (bdir ? LV_BDIR : 0);
+          t2lo = cb; t2hi = ce;
+          cb = blo; ce = bhi;
+          valid = ce > cb;
+        }
+      }
     }
     const unsigned long long bm = __ballot(valid);
     const int n = __popcll(bm);
     if (ntop + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
     if (valid) {
       int w = ntop + __popcll(bm & lt_mask);
-      f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | (ridx << 16);
+      f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | swc | (ridx << 16);
+      if (T2 && swc) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
 #pragma unroll
       for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
       if constexpr (ENUM) {
@@ -899,28 +1084,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       sbn = 0;
       if (j == last) {
         if (has) {
-          const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
-          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max);
+          const CandList LL = cand_start<SP>(g, md, j + 1, m, pos, mt);
+          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, mt);
           st_leaves++;
         }
         __syncthreads();
         continue;
       }
       int cb = 0, ce = 0, cdir = 0;
+      int swc = 0, t2lo = 0, t2hi = 0;
       bool valid = has;
       if (valid) {
-        const CandList LL = cand_start<SP>(g, md, j + 1, m, pos);
+        const CandList LL = cand_start<SP>(g, md, j + 1, m, pos, mt);
         cb = LL.lo;
         cdir = LL.dir;
-        ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend);
+        ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend, mt);
         valid = ce > cb;
+        if (T2 && valid && j == M - 3 && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
+          int blo, bhi, bdir;
+          if (mt) meter.b += 4;
+          tail2_range<SP>(g, md, m, LL.pos[cb], pend, &blo, &bhi, &bdir, mt);
+          if (bhi - blo < ce - cb) {
+            swc = LV_SWAP | (bdir ? LV_BDIR : 0);
+            t2lo = cb; t2hi = ce;
+            cb = blo; ce = bhi;
+            valid = ce > cb;
+          }
+        }
       }
       const unsigned long long bm = __ballot(valid);
       const int n = __popcll(bm);
       if (top + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
       if (valid) {
         const int w = top + __popcll(bm & lt_mask);
-        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | (ridx << 16);
+        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | swc | (ridx << 16);
+        if (T2 && swc) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
 #pragma unroll
         for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
       }
@@ -947,7 +1145,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       want = uni(__shfl(want, 0));
       for (int d = 0; d < want; d++) {
         // split the shallowest frame with at least 2 * steal_min candidates left
-        const bool can = lane < top && (f_end[lane] - f_cur[lane] >= 2 * args.steal_min);
+        // (a swapped frame keeps its edge M-2 range in this wavefront's side table: not donated)
+        const bool can = lane < top && !(f_lvl[lane] & LV_SWAP) && (f_end[lane] - f_cur[lane] >= 2 * args.steal_min);
         const unsigned long long cm = __ballot(can);
         if (cm == 0ull) break;
         const int b = __ffsll((long long)cm) - 1;
@@ -1028,7 +1227,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
     const unsigned long long leaves = wave_sum64(st_leaves);
     if (lane == TM_STAT_LEAVES) v = leaves;
-    if constexpr (INSTR) {
+    if constexpr (INSTR == 2) {
+      const unsigned long long a = wave_sum64(meter.b);
+      if (lane == TM_STAT_REF_X1) v = a;  // bytes read by the search (metered variant)
+    }
+    if constexpr (INSTR == 1) {
       const unsigned long long a = wave_sum64(rx1), b = wave_sum64(rx2), c = wave_sum64(rp);
       if (lane == TM_STAT_REF_X1) v = a;
       if (lane == TM_STAT_REF_X2) v = b;

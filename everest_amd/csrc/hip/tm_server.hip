@@ -246,14 +246,27 @@ void complete(tm_server *s, int k) {
     r.status = st;
     r.count = ln.h_counts[i];
     if (ln.enumerate) {
-      const int64_t nrows = st == TM_OK ? std::min<int64_t>((int64_t)r.count, r.cap) : 0;
-      r.rows.resize((size_t)nrows * r.M);
-      if (nrows > 0) {
-        hipError_t e = hipMemcpy(r.rows.data(), s->arena.base + r.arena_off, sizeof(int32_t) * r.rows.size(),
-                                 hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-          r.status = lane_error("server rows", e);
+      if (st == TM_OK && r.cap > 0 && r.count >= (uint64_t)r.cap) {
+        // at the cap: the deterministic rows, as tm_enum returns them (the
+        // launches run on this lane's slot, the other lanes go on meanwhile)
+        r.rows.resize((size_t)r.cap * r.M);
+        int64_t nw = 0;
+        const tm_status cs = enum_cut(s->g, sl, r.M, r.src, r.dst, r.delta, 0, s->g->E - r.M + 1, r.cap, nullptr,
+                                      r.rows.data(), &r.count, &nw);
+        if (cs != TM_OK) {
+          r.status = cs;
           r.rows.clear();
+        }
+      } else {
+        const int64_t nrows = st == TM_OK ? std::min<int64_t>((int64_t)r.count, r.cap) : 0;
+        r.rows.resize((size_t)nrows * r.M);
+        if (nrows > 0) {
+          hipError_t e = hipMemcpy(r.rows.data(), s->arena.base + r.arena_off, sizeof(int32_t) * r.rows.size(),
+                                   hipMemcpyDeviceToHost);
+          if (e != hipSuccess) {
+            r.status = lane_error("server rows", e);
+            r.rows.clear();
+          }
         }
       }
       s->arena.release(r.arena_off, r.arena_len);

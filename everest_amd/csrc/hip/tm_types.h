@@ -187,4 +187,7 @@ typedef struct {
   int idle_cap;               // most wavefronts kept waiting on tickets without records
   int steal_min;              // a donated half holds at least this many candidates
   long long steal_after_ticks; // a task donates only after running this long (wall clock)
+  int t2_min;                 // counting: frames of edge M-2 with at least this many candidates are summed
+                              // over the last edge's candidates when that list is shorter (two-edge tail)
+  int pad_t2;
 } tm_kernel_args;

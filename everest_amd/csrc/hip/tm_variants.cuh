@@ -10,7 +10,7 @@
 #define TMV_DEF(N, E, I, W, M)                                                                              \
   tm_status tmv::TMV_NAME(N, E, I, W, M)(bool occ, hipStream_t stream, int lds, int *blocks,                \
                                          const tm_kernel_args *a, int grid) {                               \
-    auto k = tmk::mine_kernel<TM_MAXE, N, (E) != 0, (I) != 0, tmk::DynSpec, W, (M) != 0>;                   \
+    auto k = tmk::mine_kernel<TM_MAXE, N, (E) != 0, (I), tmk::DynSpec, W, (M) != 0>;                   \
     hipError_t e;                                                                                           \
     if (occ) {                                                                                              \
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k, 64, lds);                                 \

@@ -1,7 +1,8 @@
 // tm_variants.h -- the mining kernel's instantiations, each behind a plain
 // function (defined in the tm_kv_*.hip units, compiled in parallel).
 //
-// A variant is named by node-map width (4 or 8), enumerate, instrumented,
+// A variant is named by node-map width (4 or 8), enumerate, instrumented (1:
+// reference work, 2: metered bytes),
 // register cap (1: compiler's choice, 5: 96 VGPRs, 6: 80 VGPRs) and
 // single/multi-query form.  occ = true: write the resident workgroups per CU
 // for `lds` bytes of dynamic LDS to *blocks; otherwise launch `grid`
@@ -29,6 +30,7 @@ TMV_DECL(4, 0, 0, 5, 1)
 TMV_DECL(4, 0, 0, 6, 1)
 TMV_DECL(4, 0, 1, 1, 0)
 TMV_DECL(4, 1, 0, 1, 0)
+TMV_DECL(4, 0, 2, 1, 0)
 TMV_DECL(8, 0, 0, 1, 0)
 TMV_DECL(8, 0, 0, 5, 0)
 TMV_DECL(8, 0, 0, 6, 0)
@@ -37,15 +39,19 @@ TMV_DECL(8, 0, 0, 5, 1)
 TMV_DECL(8, 0, 0, 6, 1)
 TMV_DECL(8, 0, 1, 1, 0)
 TMV_DECL(8, 1, 0, 1, 0)
+TMV_DECL(8, 0, 2, 1, 0)
 
-// The variant for a launch shape (nmap 4 or 8; wpe 1, 5 or 6).
-inline variant_fn pick(int nmap, bool enumerate, bool instr, int wpe, bool multi) {
+// The variant for a launch shape (nmap 4 or 8; instr 0 counting, 1 reference
+// work, 2 metered; wpe 1, 5 or 6).
+inline variant_fn pick(int nmap, bool enumerate, int instr, int wpe, bool multi) {
   if (nmap == 4) {
+    if (instr == 2) return TMV_NAME(4, 0, 2, 1, 0);
     if (instr) return TMV_NAME(4, 0, 1, 1, 0);
     if (enumerate) return TMV_NAME(4, 1, 0, 1, 0);
     if (multi) return wpe == 6 ? TMV_NAME(4, 0, 0, 6, 1) : (wpe == 5 ? TMV_NAME(4, 0, 0, 5, 1) : TMV_NAME(4, 0, 0, 1, 1));
     return wpe == 6 ? TMV_NAME(4, 0, 0, 6, 0) : (wpe == 5 ? TMV_NAME(4, 0, 0, 5, 0) : TMV_NAME(4, 0, 0, 1, 0));
   }
+  if (instr == 2) return TMV_NAME(8, 0, 2, 1, 0);
   if (instr) return TMV_NAME(8, 0, 1, 1, 0);
   if (enumerate) return TMV_NAME(8, 1, 0, 1, 0);
   if (multi) return wpe == 6 ? TMV_NAME(8, 0, 0, 6, 1) : (wpe == 5 ? TMV_NAME(8, 0, 0, 5, 1) : TMV_NAME(8, 0, 0, 1, 1));

@@ -79,6 +79,10 @@ typedef struct tm_tuning {
                             step expands its own survivors; 0 = the library's choice */
   int32_t sb_inline;     /* counting: with nothing buffered, a step with at least this many survivors
                             (1..64) expands them itself; 0 = the library's choice */
+  int32_t tail_swap;     /* counting, two-edge tail: when the last motif edge does not touch the node the
+                            second-to-last one brings, a partial match whose second-to-last edge has at
+                            least this many candidates in the window is counted over the last edge's
+                            candidates instead, if those are fewer; 0 = library default (32), < 0 = never */
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */
@@ -185,18 +189,39 @@ tm_status tm_count_work(tm_graph *g, int32_t num_edges, const int32_t *msrc, con
                         int32_t delta, int32_t beg, int32_t end, const tm_tuning *tune,
                         uint64_t *count, uint64_t *work);
 
+/*
+ * tm_count plus the algorithmic bytes of this library's own search for the
+ * same query: a metered copy of the counting kernel follows the same search
+ * (root gap filter, pruning rules, two-edge tail, last-edge range
+ * arithmetic) and adds up the bytes each of its memory reads needs: 16 per
+ * lane and gap rule for a block of 4 roots, 12 per root edge record, 4 per
+ * int32 word of a candidate (position, neighbour, first-candidate times) and
+ * per probe of every binary or galloping search.  Launch parameters as
+ * tm_count (register caps do not apply: one variant).  This is the unit
+ * behind the roofline of the counting kernel: bytes / kernel time.
+ * No reference counterpart (measurement).
+ */
+tm_status tm_count_bytes(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                         int32_t delta, int32_t beg, int32_t end, const tm_tuning *tune,
+                         uint64_t *count, uint64_t *bytes);
+
 /* Duration in ms of the graph's last mining kernel, from HIP events recorded
  * around it on the library's stream.  No reference counterpart (measurement). */
 tm_status tm_last_kernel_ms(const tm_graph *g, double *ms);
 
 /*
  * Enumerate matches: writes up to `cap` rows of num_edges int32 original line
- * numbers (Euid of p_0 .. p_{M-1}) to `rows` (host, cap * num_edges ints) and
- * the number of matches found to *count.  When *count <= cap every match is
- * written; beyond the cap, dispatch of new roots stops and which rows are
- * returned depends on the schedule, as in the reference
- * (system/tmine/KernelGen.py:509-528, system/templates/enumerate/func.snp).
- * *rows_written = min(*count, cap).
+ * numbers (Euid of p_0 .. p_{M-1}) to `rows` (host, cap * num_edges ints).
+ * When the query has fewer than `cap` matches every match is written
+ * (*count = *rows_written = the number of matches).  At the cap the
+ * reference stops handing out roots and returns whichever rows the schedule
+ * produced (system/tmine/KernelGen.py:509-528,
+ * system/templates/enumerate/func.snp); here the rows are deterministic
+ * instead: with r* the first root at which the cumulative count over
+ * [beg, r*] reaches `cap`, every row of the roots [beg, r*) and, from root
+ * r*'s rows, the smallest in lexicographic order of their Euid tuples, cap
+ * rows in all; *count = the matches of the roots [beg, r*] (>= cap).
+ * *rows_written = min(matches, cap).
  */
 tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
                   int32_t delta, int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune,
@@ -281,7 +306,8 @@ tm_status tm_batch_free(tm_batch *b);
  * once its row buffer (enum_cap * num_edges int32) fits in the enumeration
  * arena.  Launches on different slots run concurrently on the device, so
  * requests that arrive while others run overlap with them.  Each result is
- * the same as tm_count / tm_enum of the request alone.  While a server
+ * the same as tm_count / tm_enum of the request alone (rows at the cap
+ * included: they are chosen as tm_enum chooses them).  While a server
  * exists, other calls on its graph wait.
  */
 typedef struct tm_server tm_server;

@@ -43,3 +43,22 @@ def all_3edge_motifs():
                 pass
             out.append([(0, 1, 0), (e1[0], e1[1], 1), (e2[0], e2[1], 2)])
     return out
+
+
+def deterministic_cut(g, all_rows, cap):
+    """The rows tm_enum returns at a cap, from every row of a query in root
+    order (oracle.enum with a cap above the total): all rows of the roots
+    before r* (the first root at which the cumulative count reaches the cap)
+    and, of root r*'s rows, the smallest by their Euid tuples.  Returns
+    (rows as a list of tuples, count of the roots up to r*)."""
+    rows = [tuple(r) for r in all_rows.tolist()]
+    if len(rows) <= cap:
+        return rows, len(rows)
+    pos_of = np.empty(len(g["Euid"]), dtype=np.int64)
+    pos_of[np.asarray(g["Euid"])] = np.arange(len(g["Euid"]))
+    roots = [int(pos_of[r[0]]) for r in rows]
+    assert roots == sorted(roots), "oracle rows are not in root order"
+    rstar = roots[cap - 1]
+    head = [r for r, p in zip(rows, roots) if p < rstar]
+    star = sorted(r for r, p in zip(rows, roots) if p == rstar)
+    return head + star[:cap - len(head)], len(head) + len(star)

@@ -431,8 +431,59 @@ def ref_probes(g, md, k, m, p, pend):
     return (e - lo) + (1 if e < hi else 0)
 
 
+SWAPS = [0]  # swapped frames decided so far (tests read it)
+
+
+def _swap_frame(g, md, f, t2_min, frames, w):
+    """The kernel's two-edge-tail decision at a push of a frame of edge
+    M-2 (stack index w): returns the frame to push (swapped to the last
+    edge's candidates when those are fewer), or None when no match can
+    follow.  Checks that the side-table entry w mod 64 is not in use by
+    another live swapped frame."""
+    nA = f["end"] - f["cur"]
+    if t2_min is None or nA < t2_min or not tail2_eligible(md):
+        return f
+    pos, nbr = _list_arrays(g, md, md["M"] - 2, f["dir"])
+    A = (pos, nbr, f["cur"], f["end"], f["dir"])
+    bpos, bnbr, blo, bhi, bdir = tail2_lists(g, md, f["m"], A, f["pend"])
+    if bhi - blo >= nA:
+        return f
+    SWAPS[0] += 1
+    if bhi <= blo:
+        return None
+    for i in range(w):
+        fi = frames[i]
+        if fi is not None and fi.get("swap") and i % WAVE == w % WAVE:
+            raise AssertionError("side-table entry %d in use by frame %d" % (w % WAVE, i))
+    return dict(f, cur=blo, end=bhi, swap=True, alo=f["cur"], ahi=f["end"], bdir=bdir)
+
+
+def _list_arrays(g, md, j, direction):
+    kind = md["kind"][j]
+    if kind == K_NEW_DST:
+        return g["outPos"], g["outNbr"]
+    if kind == K_NEW_SRC:
+        return g["inPos"], g["inNbr"]
+    return (g["inPairPos"] if direction else g["outPairPos"]), None
+
+
+def _swap_item(g, md, f, slot):
+    """Count of one dealt candidate of a swapped frame (the kernel's tail2_item)."""
+    M = md["M"]
+    pos, nbr = _list_arrays(g, md, M - 2, f["dir"])
+    A = (pos, nbr, f["alo"], f["ahi"], f["dir"])
+    kk = md["kind"][M - 1]
+    if kk == K_NEW_DST:
+        qpos, z = g["outPos"][slot], g["outNbr"][slot]
+    elif kk == K_NEW_SRC:
+        qpos, z = g["inPos"][slot], g["inNbr"][slot]
+    else:
+        qpos, z = (g["inPairPos"][slot] if f["bdir"] else g["outPairPos"][slot]), -1
+    return tail2_item(g, md, f["m"], A, qpos, z)
+
+
 def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=256, cap=1 << 30, instr=False,
-             sb_flush=SB_FLUSH, sb_inline=SB_FLUSH):
+             sb_flush=SB_FLUSH, sb_inline=SB_FLUSH, t2_min=None):
     """Simulate one wavefront mining roots [beg, end) alone; returns
     (count, rows, stats).  With instr, stats["ref"] = [R, X1, X2, P]."""
     M = md["M"]
@@ -485,6 +536,10 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                     pushes.append(dict(cur=cb, end=ce, p=r, pend=pend, lvl=0, dir=d, m=m[:], e=[r], tl=t + delta))
             assert top + len(pushes) <= FR
             for f in pushes:
+                if not enumerate_ and not instr and M == 3:
+                    f = _swap_frame(g, md, f, t2_min, frames, top)
+                    if f is None:
+                        continue
                 frames[top] = f
                 top += 1
             # ---- expansion steps until the stack and the survivor buffer drain
@@ -509,9 +564,13 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                                                  e=[], tl=f["tl"]))
                     sb = []
                     assert top + len(children) <= FR, "frame stack overflow"
-                    for k, ch in enumerate(children):
-                        frames[top + k] = ch
-                    top += len(children)
+                    for ch in children:
+                        if j == M - 3:
+                            ch = _swap_frame(g, md, ch, t2_min, frames, top)
+                            if ch is None:
+                                continue
+                        frames[top] = ch
+                        top += 1
                     continue
                 L = Ltop
                 lim = WAVE - len(sb) if compact else WAVE
@@ -551,6 +610,9 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                 children = []
                 cur = []  # survivors of this step (compaction)
                 for slot, f in items:
+                    if f.get("swap"):
+                        cnt += _swap_item(g, md, f, slot)
+                        continue
                     m = list(f["m"])
                     if kind == K_NEW_DST:
                         pos, node = g["outPos"][slot], g["outNbr"][slot]
@@ -615,9 +677,14 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                             children.append(dict(cur=Lc[2], end=ce, p=pos, pend=f["pend"], lvl=j, dir=Lc[4], m=m,
                                                  e=[], tl=f["tl"]))
                 assert ntop + len(children) <= FR, "frame stack overflow"
-                for k, ch in enumerate(children):
-                    frames[ntop + k] = ch
-                top = ntop + len(children)
+                top = ntop
+                for ch in children:
+                    if not enumerate_ and not instr and j == M - 3:
+                        ch = _swap_frame(g, md, ch, t2_min, frames, top)
+                        if ch is None:
+                            continue
+                    frames[top] = ch
+                    top += 1
     return cnt, rows, stats
 
 
@@ -889,3 +956,112 @@ def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, ste
     assert len(sh.taken) == sh.tail == sh.ndonated, "a donated record was never taken"
     assert sh.busy == 0 and sh.idle == 0 and sh.waiters == 0
     return sum(w.cnt for w in waves), sh.ndonated
+
+
+# ------------------------------------------------ two-edge tail by the shorter list
+
+def tail2_eligible(md):
+    """The last edge k = M-1 does not touch the node edge j = M-2 introduces
+    (if it introduces one): given the partial match through edge j - 1, the
+    last edge's candidate list is then fixed, and only the position order
+    (and, when both edges bring new nodes, their distinctness) ties the two."""
+    M = md["M"]
+    if M < 3:
+        return False
+    j, k = M - 2, M - 1
+    if md["kind"][j] == K_CLOSE:
+        return True
+    w = md["mapped"][j]  # the node edge j introduces
+    return md["a"][k] != w and md["b"][k] != w
+
+
+def tail2_lists(g, md, m, A, pend):
+    """A = edge j's list (pos, nbr, lo, hi, dir) bounded by the window ([lo, hi)
+    all inside (p_{j-1}, pend)); returns edge k's list restricted to positions
+    in (pos of A's first slot, pend): (pos, nbr, lo, hi, dir)."""
+    M = md["M"]
+    k = M - 1
+    pa0 = A[0][A[2]]
+    kind = md["kind"][k]
+    if kind == K_NEW_DST:
+        x = m[md["a"][k]]
+        lo = upper_bound(g["outPos"], g["outR"][x], g["outR"][x + 1], pa0)
+        hi = gallop_lb(g["outPos"], lo, g["outR"][x + 1], pend)
+        return g["outPos"], g["outNbr"], lo, hi, 0
+    if kind == K_NEW_SRC:
+        y = m[md["b"][k]]
+        lo = upper_bound(g["inPos"], g["inR"][y], g["inR"][y + 1], pa0)
+        hi = gallop_lb(g["inPos"], lo, g["inR"][y + 1], pend)
+        return g["inPos"], g["inNbr"], lo, hi, 0
+    x, y = m[md["a"][k]], m[md["b"][k]]
+    olo, ohi = g["outR"][x], g["outR"][x + 1]
+    ilo, ihi = g["inR"][y], g["inR"][y + 1]
+    if ohi - olo <= ihi - ilo:
+        s0 = lower_bound(g["outPairNbr"], olo, ohi, y)
+        s1 = gallop_ub(g["outPairNbr"], s0, ohi, y)
+        pos, d = g["outPairPos"], 0
+    else:
+        s0 = lower_bound(g["inPairNbr"], ilo, ihi, x)
+        s1 = gallop_ub(g["inPairNbr"], s0, ihi, x)
+        pos, d = g["inPairPos"], 1
+    lo = upper_bound(pos, s0, s1, pa0)
+    return pos, None, lo, gallop_lb(pos, lo, s1, pend), d
+
+
+def tail2_item(g, md, m, A, q_pos, q_nbr):
+    """Matches of the last two edges whose last edge is the candidate at
+    position q_pos (new endpoint q_nbr when edge k brings a node): the
+    candidates c of edge j before it (position < q_pos) whose new node, if
+    any, is not mapped and differs from q_nbr."""
+    M = md["M"]
+    j, k = M - 2, M - 1
+    nm = md["mapped"][j]
+    if md["kind"][k] != K_CLOSE and q_nbr in m[:nm]:
+        return 0
+    pos, nbr, lo, hi, _ = A
+    r = lower_bound(pos, lo, hi, q_pos) - lo
+    if r <= 0 or md["kind"][j] == K_CLOSE:
+        return max(r, 0)
+    pa0 = pos[lo]
+    if md["kind"][j] == K_NEW_DST:
+        base = m[md["a"][j]]
+        pn, pp, plo, phi = g["outPairNbr"], g["outPairPos"], g["outR"][base], g["outR"][base + 1]
+    else:
+        base = m[md["b"][j]]
+        pn, pp, plo, phi = g["inPairNbr"], g["inPairPos"], g["inR"][base], g["inR"][base + 1]
+    for i in range(nm):
+        r -= pair_count(pn, pp, plo, phi, m[i], pa0 - 1, q_pos)
+    if md["kind"][k] != K_CLOSE:
+        r -= pair_count(pn, pp, plo, phi, q_nbr, pa0 - 1, q_pos)
+    return r
+
+
+def tail2_count(g, md, m, A, pend):
+    """Matches of the last two motif edges for one frame of edge j (list A
+    window-bounded), summed over the last edge's candidates instead of edge
+    j's: the kernel takes this way when edge k's window list is the shorter."""
+    B = tail2_lists(g, md, m, A, pend)
+    pos, nbr, lo, hi, _ = B
+    return sum(tail2_item(g, md, m, A, pos[s], nbr[s] if nbr is not None else -1) for s in range(lo, hi))
+
+
+def tail2_direct(g, md, m, A, pend, scan_max=16):
+    """The same sum the way the kernel takes it otherwise: every candidate c
+    of edge j, then the last edge's leaf count."""
+    M = md["M"]
+    j = M - 2
+    pos, nbr, lo, hi, d = A
+    total = 0
+    for s in range(lo, hi):
+        mm = list(m)
+        if md["kind"][j] == K_NEW_DST:
+            if nbr[s] in m[:md["mapped"][j]]:
+                continue
+            mm[md["b"][j]] = nbr[s]
+        elif md["kind"][j] == K_NEW_SRC:
+            if nbr[s] in m[:md["mapped"][j]]:
+                continue
+            mm[md["a"][j]] = nbr[s]
+        Lc = cand_start(g, md, j + 1, mm, pos[s])
+        total += leaf_count(g, md, j + 1, mm, Lc, pos[s], pend, scan_max)
+    return total

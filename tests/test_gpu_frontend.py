@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle, prep
+from helpers import deterministic_cut
 
 pytestmark = pytest.mark.gpu
 
@@ -56,9 +57,10 @@ def test_generated_worker_counts(inputs):
 
 
 def test_generated_worker_enumerates(inputs):
-    """Below the cap the rows are exactly the oracle's; above it (the reference
-    stops dispatching roots once the cap is claimed, so which rows come back
-    depends on the schedule) they are distinct rows of the full set."""
+    """Below the cap the rows are exactly the oracle's; at the cap (where the
+    reference returns whichever rows its schedule produced) they are the
+    deterministic cut: the rows of the roots before the root that reaches
+    the cap, then that root's smallest rows."""
     from everest_amd.tmine import frontend
     from oracle import motif as omotif
     root, g = inputs
@@ -75,9 +77,9 @@ def test_generated_worker_enumerates(inputs):
     q = _query(root, "qe2.yaml", "cycle3.txt", 1200, "  enumerate: %d\n" % cap)
     say("frontend enumerate cycle3 1200 cap", cap)
     cn, _, _, rows = frontend.MiningSingle(q).run()
-    got = list(map(tuple, rows.tolist()))
-    assert cn >= cap and len(got) == cap
-    assert len(set(got)) == cap and set(got) <= want_set
+    want_rows, want_cnt = deterministic_cut(g, want, cap)
+    assert cn == want_cnt
+    assert sorted(map(tuple, rows.tolist())) == sorted(want_rows)
 
 
 def test_stock_plugin_and_executions(inputs):

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle, prep
-from helpers import random_graph, random_motif, all_3edge_motifs
+from helpers import random_graph, random_motif, all_3edge_motifs, deterministic_cut
 
 pytestmark = pytest.mark.gpu
 
@@ -52,16 +52,32 @@ def test_count_small_random(small_cases):
 
 
 def test_enum_small_random(small_cases):
+    """Below the cap the row set is the oracle's; at a cap of a third or a
+    tenth of the matches it is the deterministic cut (every row of the roots
+    before the root that reaches the cap, then that root's smallest rows),
+    whatever the schedule: two runs with different launch shapes agree."""
     abi = _abi()
+    beyond = 0
     for (i, U, V, T, mot, delta) in small_cases[:30]:
         g = prep.preprocess(U, V, T)
         total, rows = oracle.enum(g, mot, delta, 1 << 20)
         say("enum case", i, "motif", mot, "delta", delta, "total", total)
         dg = abi.DeviceGraph.from_dict(g)
-        cnt, grows = dg.enum(mot, delta, cap=max(total, 1))
-        dg.close()
+        cnt, grows = dg.enum(mot, delta, cap=max(total, 1) + 1)
         assert cnt == total
         assert sorted(map(tuple, grows.tolist())) == sorted(map(tuple, rows.tolist()))
+        for cap in sorted({max(1, total // 3), max(1, total // 10), total}):
+            if total < 2:
+                continue
+            want, want_cnt = deterministic_cut(g, rows, cap)
+            for tune in (None, abi.tuning(waves_per_cu=1, claim_blocks=1)):
+                cnt, grows = dg.enum(mot, delta, cap=cap, tune=tune)
+                got = sorted(map(tuple, grows.tolist()))
+                assert got == sorted(want), (i, cap)
+                assert cnt == want_cnt, (i, cap, cnt, want_cnt)
+            beyond += cap < total
+        dg.close()
+    assert beyond > 10
 
 
 def test_batch_equals_single(small_cases):
@@ -267,27 +283,25 @@ def test_server_alone_and_in_mix():
         d = int(rng.integers(0, 400))
         cap = int(rng.integers(0, 5000)) if k % 5 == 0 else -1
         reqs.append((mot, d, cap))
-    want = []  # (exact count, rows when the cap holds them all)
+    want = []  # (exact count, (count, rows) of tm_enum at the request's cap)
     for mot, d, cap in reqs:
         c = dg.count(mot, d)
-        want.append((c, dg.enum(mot, d, cap=max(c, 1))[1] if 0 <= cap and c <= cap else None))
+        want.append((c, dg.enum(mot, d, cap=cap) if cap >= 0 else None))
 
     def check(k, r, srv, t):
         mot, d, cap = reqs[k]
         total, rows_want = want[k]
         assert r["status"] == 0, (k, r)
-        if cap < 0 or total <= cap:
+        if cap < 0 or total < cap:
             assert r["count"] == total, (k, r, total)
-        else:  # the enumeration stops handing out roots once the cap is claimed
-            assert r["count"] >= cap, (k, r, cap)
+        else:  # at the cap: the deterministic cut, as tm_enum returns it
+            assert r["count"] == rows_want[0], (k, r, rows_want[0])
         if cap < 0:
             srv.release(t)
             return
         rows = srv.take_rows(t, len(mot), cap)
-        if total <= cap:
-            assert sorted(map(tuple, rows.tolist())) == sorted(map(tuple, rows_want.tolist()))
-        else:
-            assert len(rows) == cap and len(set(map(tuple, rows.tolist()))) == cap
+        want_rows = rows_want[1]
+        assert sorted(map(tuple, rows.tolist())) == sorted(map(tuple, want_rows.tolist())), k
 
     say("server: alone")
     with abi.Server(dg, slots=4, max_batch=8) as srv:
@@ -346,3 +360,25 @@ def test_prepared_batch_graph_replays():
     pb.close()
     assert [oracle.count(g, m, d, nthreads=8) for (m, d) in qs[:10]] == want[:10]
     dg.close()
+
+
+def test_two_edge_tail_and_meter(small_cases):
+    """The two-edge tail (summing the last two edges over the last edge's
+    candidates when that list is shorter) forced on every frame it applies
+    to, and switched off: counts equal the oracle either way; the metered
+    kernel (bytes of the search) counts the same."""
+    abi = _abi()
+    for (i, U, V, T, mot, delta) in small_cases:
+        if len(mot) < 3:
+            continue
+        g = prep.preprocess(U, V, T)
+        say("two-edge tail case", i, "motif", mot, "delta", delta)
+        want = oracle.count(g, mot, delta, nthreads=4)
+        dg = abi.DeviceGraph.from_dict(g)
+        for ts in (1, -1, 0):
+            got = dg.count(mot, delta, tune=abi.tuning(tail_swap=ts))
+            assert got == want, (i, ts, got, want)
+        c, b = dg.count_bytes(mot, delta, tune=abi.tuning(tail_swap=1))
+        assert c == want and (b > 0 or want == 0), (i, c, b)
+        assert dg.batch_count([(mot, delta)] * 3, tune=abi.tuning(tail_swap=1)) == [want] * 3
+        dg.close()

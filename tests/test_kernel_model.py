@@ -120,3 +120,27 @@ def test_model_batch_dispatch_covers_every_match_once():
         ipairs = km.batch_dispatch(gv, qs, instr=True)
         for qi, (md, d, b, e) in enumerate(qs):
             assert sorted(r for (q, r) in ipairs if q == qi) == list(range(b, e))
+
+
+def test_model_two_edge_tail_swap():
+    """Frames of the second-to-last edge summed over the last edge's
+    candidates (swapped when that list is shorter): counts equal the oracle
+    with and without survivor compaction, at several thresholds, and the
+    side table entry of a swapped frame is never shared by a live one."""
+    rng = np.random.default_rng(9)
+    km.SWAPS[0] = 0
+    for _ in range(60):
+        E = int(rng.integers(20, 300))
+        U, V, T = random_graph(rng, E, int(rng.integers(3, 10)), int(rng.integers(2, 60)))
+        M = int(rng.integers(3, 7))
+        mot = random_motif(rng, M, max_nodes=int(rng.integers(3, 6)))
+        md = km.build_motif([(a, b) for a, b, _ in mot])
+        delta = int(rng.integers(0, 50))
+        g = prep.preprocess(U, V, T)
+        gv = km.build_view(g)
+        want = oracle.count(g, mot, delta, nthreads=1)
+        for t2 in (1, 2, 8):
+            got, _, _ = km.run_wave(gv, md, delta, 0, g["E"] - M + 1, t2_min=t2,
+                                    sb_flush=int(rng.choice([0, 1, 32])), sb_inline=int(rng.choice([1, 32])))
+            assert got == want, (mot, delta, t2)
+    assert km.SWAPS[0] > 100
