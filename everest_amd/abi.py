@@ -89,7 +89,7 @@ def lib():
         L.tm_server_stats.argtypes = [P, P, I32]
         L.tm_server_destroy.argtypes = [P]
         for name in ("tm_graph_upload", "tm_graph_free", "tm_graph_info", "tm_count", "tm_enum",
-                     "tm_batch_count", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work",
+                     "tm_batch_count", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work", "tm_count_bytes",
                      "tm_batch_prepare", "tm_batch_run", "tm_batch_free", "tm_shape_class", "tm_set_class_tuning",
                      "tm_server_create", "tm_server_submit", "tm_server_wait", "tm_server_take_rows",
                      "tm_server_release", "tm_server_stats", "tm_server_destroy"):

@@ -235,13 +235,23 @@ namespace tmh {
 
 tm_status set_error(tm_status s, const std::string &msg) { return fail(s, msg); }
 
-static int shape_class_of(int M, const int32_t *src, const int32_t *dst) {
+int shape_class_of(int M, const int32_t *src, const int32_t *dst) {
   int n = 0;
   for (int k = 0; k < M; k++) n = std::max(n, 1 + std::max(src[k], dst[k]));
   return std::max(1, M - 2) * 2 + (n <= 4 ? 0 : 1);
 }
 
 // The tuning table entry of a motif's shape class, if one is installed.
+// The launch parameters of a call: its own, unless it brings none (a null
+// pointer or every field 0, both meaning "the library's choice"), in which
+// case the tuning installed for the motif's shape class, if any.
+const tm_tuning *tuning_for_class(tm_graph *g, int cls, const tm_tuning *tune) {
+  static const tm_tuning none{};
+  if (tune && memcmp(tune, &none, sizeof(none)) != 0) return tune;
+  if (cls >= 0 && cls < TM_NCLASSES && g->class_set[cls]) return &g->class_tune[cls];
+  return nullptr;
+}
+
 const tm_tuning *class_tuning_of(tm_graph *g, int M, const int32_t *src, const int32_t *dst) {
   const int cls = shape_class_of(M, src, dst);
   return (cls >= 0 && cls < TM_NCLASSES && g->class_set[cls]) ? &g->class_tune[cls] : nullptr;
@@ -527,7 +537,7 @@ tm_status enqueue(tm_graph *g, Slot &sl, const Plan &pl, bool timing, bool count
   tm_status st;
   // the generation count belongs to the graph's slot (sl may be a view of it
   // with other staging arrays); a captured launch is counted at each replay
-  if (count && (st = count_launch(g, g->slot[sl.index], sl.stream, 1)) != TM_OK) return st;
+  if (count && (st = count_launch(g, sl.own_ring ? sl : g->slot[sl.index], sl.stream, 1)) != TM_OK) return st;
   hipLaunchKernelGGL(reset_kernel, dim3((pl.nq + 255) / 256), dim3(256), 0, sl.stream, pl.a.sched, pl.d_counts,
                      pl.d_stops, pl.nq, pl.grid);
   HIPTRY(hipGetLastError());
@@ -636,14 +646,17 @@ tm_status stage_queries(tm_graph *g, Slot &sl, int q0, int nq, const int *idx, c
 // Check a completed launch of the slot: its scheduler state hs (copied back)
 // gives the launch statistics and error bits.
 tm_status collect(tm_graph *g, Slot &sl, const tm_sched &hs) {
-  memcpy(sl.last_stats, hs.stats, sizeof(sl.last_stats));
-  sl.last_stats[TM_STAT_GRID] = (unsigned long long)sl.last_grid;
-  sl.last_stats[TM_STAT_CLAIM] = (unsigned long long)sl.last_claim;
-  sl.last_stats[TM_STAT_LDS] = (unsigned long long)sl.last_lds;
-  sl.last_stats[TM_STAT_WPE] = (unsigned long long)sl.last_wpe;
+  // the statistics land in the graph's slot of that index (sl may be a view
+  // of it with other device buffers)
+  Slot &dst = g->slot[sl.index];
+  memcpy(dst.last_stats, hs.stats, sizeof(dst.last_stats));
+  dst.last_stats[TM_STAT_GRID] = (unsigned long long)sl.last_grid;
+  dst.last_stats[TM_STAT_CLAIM] = (unsigned long long)sl.last_claim;
+  dst.last_stats[TM_STAT_LDS] = (unsigned long long)sl.last_lds;
+  dst.last_stats[TM_STAT_WPE] = (unsigned long long)sl.last_wpe;
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) sl.last_kernel_ms = ms;
-  g->last_slot = (int)(&sl - g->slot);
+  if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) dst.last_kernel_ms = ms;
+  g->last_slot = sl.index;
   if (hs.error & ~4) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(hs.error) + ")");
   if (hs.error & 4) return fail(TM_ERR_TIMEOUT, "query exceeded its time limit; counts discarded");
   return TM_OK;
@@ -1165,16 +1178,11 @@ tm_status tm_count_bytes(tm_graph *g, int32_t num_edges, const int32_t *msrc, co
 static int shape_class(int M, int n) { return std::max(1, M - 2) * 2 + (n <= 4 ? 0 : 1); }
 
 // The graph's tuning table: a call without launch parameters of its own uses
-// the entry of its query's shape class, if one was set.
-// The launch parameters of a call: its own, unless it brings none (a null
-// pointer or every field 0, both meaning "the library's choice"), in which
-// case the tuning installed for the motif's shape class, if any.
+// the entry of its query's shape class, if one was set (tuning_for_class).
 static const tm_tuning *class_tuning(tm_graph *g, int cls, const tm_tuning *tune) {
-  static const tm_tuning none{};
-  if (tune && memcmp(tune, &none, sizeof(none)) != 0) return tune;
-  if (cls >= 0 && cls < TM_NCLASSES && g->class_set[cls]) return &g->class_tune[cls];
-  return nullptr;
+  return tuning_for_class(g, cls, tune);
 }
+
 
 static int motif_class(int32_t num_edges, const int32_t *msrc, const int32_t *mdst) {
   int n = 0;

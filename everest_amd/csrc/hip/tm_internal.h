@@ -40,6 +40,8 @@ struct Scratch {
 struct Slot {
   int index = 0;                            // position in tm_graph::slot
   int gen = 0;                              // launches since the ring was last cleared (device sched->gen)
+  bool own_ring = false;                    // a view with its own scheduler state and ring: its launches
+                                            // are counted here, not on the graph's slot of this index
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the mining kernel
   Scratch s;
@@ -112,6 +114,10 @@ tm_status collect(tm_graph *g, Slot &sl, const tm_sched &hs);
 tm_status finish(tm_graph *g, Slot &sl, int q0, int nq, uint64_t *counts);
 // The tuning table entry of a motif's shape class (nullptr when none is set).
 const tm_tuning *class_tuning_of(tm_graph *g, int M, const int32_t *src, const int32_t *dst);
+// Shape class of a motif (stack depth x node-map width), and the launch
+// parameters of a call: its own unless it brings none, else its class's entry.
+int shape_class_of(int M, const int32_t *src, const int32_t *dst);
+const tm_tuning *tuning_for_class(tm_graph *g, int cls, const tm_tuning *tune);
 // Deterministic rows of an enumeration that reaches its cap (cap > 0 rows of
 // M ints into host `rows`): every row of the roots before r*, the root at
 // which the cumulative count over [beg, r*] first reaches the cap, and the

@@ -1065,3 +1065,62 @@ def tail2_direct(g, md, m, A, pend, scan_max=16):
         Lc = cand_start(g, md, j + 1, mm, pos[s])
         total += leaf_count(g, md, j + 1, mm, Lc, pos[s], pend, scan_max)
     return total
+
+
+# ----------------------------------- persistent launches sharing the GPU's slots
+
+def run_concurrent_launches(grids, capacity, idle_cap, work, rng, max_ticks=100_000):
+    """Several persistent launches on one GPU with `capacity` resident
+    wavefront slots.  Launch L has grids[L] wavefronts; its `busy` counter
+    starts at grids[L] (a wavefront not yet started counts as holding work,
+    as the reset kernel sets it).  Each tick the hardware starts waiting
+    wavefronts of any launch (random interleaving) into free slots, and every
+    resident wavefront takes a step: it works off its roots (work[L][w]
+    steps), then goes idle (busy - 1) and either waits for donations (at most
+    idle_cap waiters per launch; a waiter holds its slot until its launch's
+    busy reaches 0) or leaves.  Returns "done" or "deadlock" (no wavefront can
+    change state and no slot is free for an unstarted one)."""
+    K = len(grids)
+    unstarted = [list(range(g)) for g in grids]
+    busy = list(grids)
+    waiters = [0] * K
+    resident = []  # [launch, wave, remaining work, state]  state: 0 work, 1 waiting
+    for tick in range(max_ticks):
+        # admit
+        free = capacity - len(resident)
+        cands = [L for L in range(K) if unstarted[L]]
+        while free > 0 and cands:
+            L = cands[int(rng.integers(0, len(cands)))]
+            w = unstarted[L].pop()
+            resident.append([L, w, work[L][w], 0])
+            free -= 1
+            cands = [L for L in range(K) if unstarted[L]]
+        if not resident and not any(unstarted):
+            return "done"
+        changed = False
+        keep = []
+        for r in resident:
+            L = r[0]
+            if r[3] == 0:
+                if r[2] > 0:
+                    r[2] -= 1
+                    changed = True
+                    keep.append(r)
+                    continue
+                busy[L] -= 1
+                changed = True
+                if waiters[L] >= idle_cap:
+                    continue  # leaves: its slot frees
+                waiters[L] += 1
+                r[3] = 1
+                keep.append(r)
+            else:
+                if busy[L] == 0:
+                    waiters[L] -= 1
+                    changed = True
+                    continue
+                keep.append(r)
+        resident = keep
+        if not changed and len(resident) >= capacity and any(unstarted):
+            return "deadlock"
+    return "timeout"

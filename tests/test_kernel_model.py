@@ -144,3 +144,27 @@ def test_model_two_edge_tail_swap():
                                     sb_flush=int(rng.choice([0, 1, 32])), sb_inline=int(rng.choice([1, 32])))
             assert got == want, (mot, delta, t2)
     assert km.SWAPS[0] > 100
+
+
+def test_model_concurrent_persistent_launches():
+    """Persistent launches running at the same time, each with a grid the size
+    of the whole GPU: with every idle wavefront waiting for donations (no
+    bound), the waiting wavefronts of the launches can fill every slot while
+    each launch still has wavefronts that never started, which hold its busy
+    count above zero: the GPU hangs (the first captured batch's failure).
+    With at most idle_cap waiters per launch and launches * idle_cap below
+    the GPU's slots, every interleaving completes."""
+    rng = np.random.default_rng(3)
+    cap = 64
+    hangs = 0
+    for trial in range(40):
+        K = int(rng.integers(2, 5))
+        grids = [cap] * K
+        work = [[int(x) for x in rng.integers(0, 6, cap)] for _ in range(K)]
+        hangs += km.run_concurrent_launches(grids, cap, 1 << 30, work, rng) == "deadlock"
+        assert km.run_concurrent_launches(grids, cap, cap // (2 * K), work, rng) == "done"
+    assert hangs > 0
+    # one launch alone never hangs, bounded or not: its whole grid is resident
+    for trial in range(20):
+        work = [[int(x) for x in rng.integers(0, 6, cap)]]
+        assert km.run_concurrent_launches([cap], cap, 1 << 30, work, rng) == "done"
