@@ -72,6 +72,7 @@ def lib():
         L.tm_enum.argtypes = [P, I32, P, P, I32, I32, I32, I64, ctypes.POINTER(Tuning), P, P,
                               ctypes.POINTER(I64)]
         L.tm_batch_count.argtypes = [P, I32, P, P, P, P, ctypes.POINTER(Tuning), P]
+        L.tm_batch_count_noalloc.argtypes = [P, I32, P, P, P, P, ctypes.POINTER(Tuning), P]
         L.tm_last_stats.argtypes = [P, P, I32]
         L.tm_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.tm_count_work.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P, P]
@@ -89,7 +90,7 @@ def lib():
         L.tm_server_stats.argtypes = [P, P, I32]
         L.tm_server_destroy.argtypes = [P]
         for name in ("tm_graph_upload", "tm_graph_free", "tm_graph_info", "tm_count", "tm_enum",
-                     "tm_batch_count", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work", "tm_count_bytes",
+                     "tm_batch_count", "tm_batch_count_noalloc", "tm_last_stats", "tm_last_kernel_ms", "tm_count_work", "tm_count_bytes",
                      "tm_batch_prepare", "tm_batch_run", "tm_batch_free", "tm_shape_class", "tm_set_class_tuning",
                      "tm_server_create", "tm_server_submit", "tm_server_wait", "tm_server_take_rows",
                      "tm_server_release", "tm_server_stats", "tm_server_destroy"):
@@ -222,6 +223,16 @@ class DeviceGraph:
         check(lib().tm_batch_count(self.handle, len(queries), _p(ne), _p(src), _p(dst), _p(dl), ctypes.byref(t),
                                    _p(out)))
         return [int(x) for x in out]
+
+    def batch_count_noalloc(self, queries, tune=None):
+        """batch_count through the entry point that allocates no device memory
+        (its buffers are a static device array; at most TM_NOALLOC_MAXQ queries)."""
+        ne, src, dst, dl = _batch_arrays(queries)
+        out = np.zeros(max(len(queries), 1), dtype=np.uint64)
+        t = tune if tune is not None else Tuning()
+        check(lib().tm_batch_count_noalloc(self.handle, len(queries), _p(ne), _p(src), _p(dst), _p(dl),
+                                           ctypes.byref(t), _p(out)))
+        return [int(x) for x in out[:len(queries)]]
 
     def set_class_tuning(self, cls, tune):
         """Install (tune: a Tuning) or clear (None) the tuning of a shape class."""

@@ -279,6 +279,26 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges,
                          const tm_tuning *tune, uint64_t *counts);
 
 /*
+ * tm_batch_count for callers that cannot allocate device memory: code under
+ * HIP graph capture, serving loops and harnesses that own every device
+ * allocation.  No counterpart in the reference.  Same queries, launches and
+ * results as tm_batch_count, for at most TM_NOALLOC_MAXQ queries per call
+ * (TM_ERR_ARG beyond).  Every device buffer of its launches (scheduler
+ * state, query tables, counts, per-wavefront counters, a donation ring of
+ * 2^15 records) lives in one static device array of
+ * TM_NOALLOC_SCRATCH_BYTES, one copy per GPU the library runs on, held
+ * until the process ends; the call makes no hipMalloc / hipFree.  Calls on
+ * one GPU share that array: this library's calls take turns (one lock per
+ * device), the array keeps the last call's contents, and launches that
+ * overlap on one GPU through another copy of the library would race.
+ */
+#define TM_NOALLOC_MAXQ 256
+#define TM_NOALLOC_SCRATCH_BYTES 7292160
+tm_status tm_batch_count_noalloc(tm_graph *g, int32_t nq, const int32_t *num_edges, const int32_t *msrc,
+                                 const int32_t *mdst, const int32_t *deltas, const tm_tuning *tune,
+                                 uint64_t *counts);
+
+/*
  * A prepared batch: the queries of tm_batch_count planned once, their tables
  * uploaded, and the whole set of per-class launches (reset, mining kernel,
  * statistics, result copies, on the graph's slot streams) captured into one
