@@ -809,10 +809,12 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
     return fail(TM_ERR_ARG, "null graph array");
   // Host-side validation: every index the kernels follow must be in range.
   const tm_edge *Eg = (const tm_edge *)hg->Eg;
+  bool self_loops = false;
   for (int p = 0; p < E; p++) {
     if (Eg[p].u < 0 || Eg[p].u >= V || Eg[p].v < 0 || Eg[p].v >= V)
       return fail(TM_ERR_ARG, "edge endpoint out of [0, V)");
     if (p && Eg[p].t < Eg[p - 1].t) return fail(TM_ERR_ARG, "edges are not sorted by time");
+    self_loops |= Eg[p].u == Eg[p].v;
   }
   for (int k = 0; k < 2; k++) {
     const int32_t *R = k ? hg->inEdgesR : hg->outEdgesR;
@@ -864,6 +866,7 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   tm_graph_view &vw = g->view;
   vw.E = E;
   vw.V = V;
+  vw.no_self_loops = self_loops ? 0 : 1;
   tm_status st;
 #define ALLOC(field, bytes)                                            \
   if ((st = dev_alloc(g, &p, (bytes))) != TM_OK) return cleanup(st); \
@@ -1211,11 +1214,14 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
     order[q] = q;
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  // fused: every query in one launch (its stack sized for the deepest class,
+  // whose tuning entry it takes), at most TM_LAUNCH_MAXQ per launch
+  const bool fuse = tune && tune->fuse_classes > 0;
   struct Group { int first, n; };
   std::vector<Group> groups;
   for (int i = 0; i < nq;) {
     int j = i;
-    while (j < nq && key[order[j]] == key[order[i]] && j - i < TM_LAUNCH_MAXQ) j++;
+    while (j < nq && (fuse || key[order[j]] == key[order[i]]) && j - i < TM_LAUNCH_MAXQ) j++;
     groups.push_back({i, j - i});
     i = j;
   }
@@ -1224,7 +1230,19 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
   for (const Group &gr : groups) {
     tm_status st = stage_queries(g, sl, 0, gr.n, order.data() + gr.first, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES,
                                  deltas, nullptr, nullptr);
-    if (st == TM_OK) st = launch(g, sl, 0, gr.n, false, class_tuning(g, key[order[gr.first]], tune), false, true);
+    const int cls = key[order[gr.first + gr.n - 1]];  // the deepest class of the launch
+    tm_tuning merged;
+    const tm_tuning *t = class_tuning(g, cls, tune);
+    if (fuse) {
+      // the deepest class's table entry, with the call's own nonzero fields on top
+      merged = (cls >= 0 && cls < TM_NCLASSES && g->class_set[cls]) ? g->class_tune[cls] : tm_tuning{};
+      const int32_t *src = (const int32_t *)tune;
+      int32_t *dst = (int32_t *)&merged;
+      for (size_t f = 0; f < sizeof(tm_tuning) / sizeof(int32_t); f++)
+        if (src[f]) dst[f] = src[f];
+      t = &merged;
+    }
+    if (st == TM_OK) st = launch(g, sl, 0, gr.n, false, t, false, true);
     if (st == TM_OK) st = finish(g, sl, 0, gr.n, out.data() + gr.first);
     if (st != TM_OK) return st;
   }

@@ -327,7 +327,8 @@ __device__ __forceinline__ CandList cand_start(const tm_graph_view &g, const tm_
 template <class SP, int MAXN>
 __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g, const tm_motif_dev &md,
                                                          int j, const int (&m)[MAXN], const CandList &L,
-                                                         int p, int pend, int scan_max, Meter *mt = nullptr) {
+                                                         int p, int pend, int scan_max, int skipx = -1,
+                                                         int zmask = 0, Meter *mt = nullptr) {
   const int b1 = gallop_lb(L.pos, L.lo, L.hi, pend, mt);
   int c = b1 - L.lo;
   if (c <= 0 || SP::kind(md, j) == TM_K_CLOSE) return c > 0 ? (unsigned long long)c : 0ull;
@@ -339,11 +340,15 @@ __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g,
     return (unsigned long long)(c - bad);
   }
   // injectivity: remove candidates whose new endpoint is already mapped
+  // (none to the list's own node in a graph without self-loops, none to
+  // skipx: the caller knows that pair has no edge left in the window)
   const bool out = SP::kind(md, j) == TM_K_NEW_DST;
   const int base = out ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j));
 #pragma unroll
-  for (int i = 0; i < MAXN; i++)
-    if (i < nm) c -= out ? edge_count(g, base, m[i], p, pend, mt) : edge_count(g, m[i], base, p, pend, mt);
+  for (int i = 0; i < MAXN; i++) {
+    if (i >= nm || ((zmask >> i) & 1) || m[i] == skipx || (g.no_self_loops && m[i] == base)) continue;
+    c -= out ? edge_count(g, base, m[i], p, pend, mt) : edge_count(g, m[i], base, p, pend, mt);
+  }
   return (unsigned long long)c;
 }
 
@@ -359,6 +364,10 @@ __device__ __forceinline__ unsigned long long leaf_count(const tm_graph_view &g,
 // The kernel takes that way when B's window part is the shorter list (hub
 // nodes: thousands of candidates for edge j against a handful for edge k),
 // and deals B's candidates to the lanes instead of A's.
+// A join: when edge j brings node w and edge k closes w with a node y mapped
+// before, the matches are the pairs (c, q) with c in A, q an edge between w
+// and y after c; summed over the edges q at y in the window (B = y's list),
+// each counts A's edges to q's other end before it (one pair search).
 
 // lvl word of a frame: level | A's pair direction << 8 | swapped << 9 |
 // B's pair direction << 10 | root index << 16
@@ -366,13 +375,22 @@ constexpr int LV_DIR = 1 << 8;
 constexpr int LV_SWAP = 1 << 9;
 constexpr int LV_BDIR = 1 << 10;
 
+enum { T2_NONE = 0, T2_INDEP = 1, T2_JOIN = 2 };
+
+template <class SP>
+__device__ __forceinline__ int tail2_mode(const tm_motif_dev &md) {
+  const int M = SP::M(md);
+  if (M < 3) return T2_NONE;
+  if (SP::kind(md, M - 2) == TM_K_CLOSE) return T2_INDEP;
+  const int w = SP::mapped(md, M - 2);  // the node edge M-2 introduces
+  const int ak = SP::a(md, M - 1), bk = SP::b(md, M - 1);
+  if (ak != w && bk != w) return T2_INDEP;
+  return SP::kind(md, M - 1) == TM_K_CLOSE ? T2_JOIN : T2_NONE;
+}
+
 template <class SP>
 __device__ __forceinline__ bool tail2_ok(const tm_motif_dev &md) {
-  const int M = SP::M(md);
-  if (M < 3) return false;
-  if (SP::kind(md, M - 2) == TM_K_CLOSE) return true;
-  const int w = SP::mapped(md, M - 2);  // the node edge M-2 introduces
-  return SP::a(md, M - 1) != w && SP::b(md, M - 1) != w;
+  return tail2_mode<SP>(md) != T2_NONE;
 }
 
 // Edge k's candidates (k = M-1) with positions in (pa0, pend), pa0 the
@@ -383,7 +401,18 @@ __device__ __forceinline__ void tail2_range(const tm_graph_view &g, const tm_mot
   const int k = SP::M(md) - 1;
   const int kind = SP::kind(md, k);
   *bdir = 0;
-  if (kind == TM_K_NEW_DST) {
+  if (tail2_mode<SP>(md) == T2_JOIN) {
+    // edges between w and y at y: y's in-list when edge k is w -> y, else its out-list
+    const int w = SP::mapped(md, k - 1);
+    const bool into_y = SP::a(md, k) == w;
+    const int y = sel(m, into_y ? SP::b(md, k) : SP::a(md, k));
+    const int *pos = into_y ? g.inPos : g.outPos;
+    const int *R = into_y ? g.inR : g.outR;
+    const int e = R[y + 1];
+    if (mt) mt->b += 8;
+    *lo = upper_bound_i(pos, R[y], e, pa0, mt);
+    *hi = gallop_lb(pos, *lo, e, pend, mt);
+  } else if (kind == TM_K_NEW_DST) {
     const int x = sel(m, SP::a(md, k));
     const int e = g.outR[x + 1];
     if (mt) mt->b += 8;
@@ -424,11 +453,27 @@ __device__ __forceinline__ void tail2_range(const tm_graph_view &g, const tm_mot
 template <class SP, int MAXN>
 __device__ __forceinline__ unsigned long long tail2_item(const tm_graph_view &g, const tm_motif_dev &md,
                                                          const int (&m)[MAXN], int alo, int ahi, int adir,
-                                                         int slot, int bdir, Meter *mt = nullptr) {
+                                                         int slot, int bdir, int zmask = 0, Meter *mt = nullptr) {
   const int M = SP::M(md);
   const int j = M - 2, k = M - 1;
   const int nm = SP::mapped(md, j);
   const int kk = SP::kind(md, k);
+  if (tail2_mode<SP>(md) == T2_JOIN) {
+    // q = an edge between y and z after A's first candidate: the candidates
+    // of A whose new node is z (edges between A's node and z) before it
+    const int w = SP::mapped(md, j);
+    const bool into_y = SP::a(md, k) == w;
+    const int qp = into_y ? g.inPos[slot] : g.outPos[slot];
+    const int zz = into_y ? g.inNbr[slot] : g.outNbr[slot];
+    if (mt) mt->b += 8;
+    if (in_first(m, nm, zz)) return 0ull;  // z must be a new node
+    const bool aout = SP::kind(md, j) == TM_K_NEW_DST;
+    const int *ap = aout ? g.outPos : g.inPos;
+    const int base = aout ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j));
+    if (mt) mt->b += 4;
+    const int p0 = ap[alo] - 1;
+    return (unsigned long long)(aout ? edge_count(g, base, zz, p0, qp, mt) : edge_count(g, zz, base, p0, qp, mt));
+  }
   int qpos, z = -1;
   if (kk == TM_K_NEW_DST) {
     qpos = g.outPos[slot];
@@ -452,10 +497,157 @@ __device__ __forceinline__ unsigned long long tail2_item(const tm_graph_view &g,
   const bool out = kj == TM_K_NEW_DST;
   const int base = out ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j));
 #pragma unroll
-  for (int i = 0; i < MAXN; i++)
-    if (i < nm) r -= out ? edge_count(g, base, m[i], p0, qpos, mt) : edge_count(g, m[i], base, p0, qpos, mt);
+  for (int i = 0; i < MAXN; i++) {
+    if (i >= nm || ((zmask >> i) & 1) || (g.no_self_loops && m[i] == base)) continue;
+    r -= out ? edge_count(g, base, m[i], p0, qpos, mt) : edge_count(g, m[i], base, p0, qpos, mt);
+  }
   if (kk != TM_K_CLOSE) r -= out ? edge_count(g, base, z, p0, qpos, mt) : edge_count(g, z, base, p0, qpos, mt);
   return (unsigned long long)r;
+}
+
+// ------------------------------------- last two edges on one list (count mode)
+//
+// When the last two motif edges j = M-2 and k = M-1 take their candidates
+// from the same list, their matches per frame of edge j (window list A,
+// K slots) have closed forms up to one term per slot:
+//   same pair, both closing (x->y twice):   C(K, 2)
+//   star (both new, same node and side):    C(K', 2) + sum_x C(n_x, 2) - sum_c sp(c)
+//   pair (new node w, then w again):        sum_c sp(c) - sum_x C(n_x, 2)
+// where x runs over the nodes mapped before edge j, n_x = slots of A whose
+// other end is x, K' = K - sum_x n_x, and sp(c) = the later slots of A with
+// the same other end as slot c (the same graph pair): its next same-pair
+// time (one coalesced load) shows it is 0 almost always.  The frame's terms
+// are added when it is pushed; its slots, dealt as usual, add or subtract
+// sp(c).  Every candidate of edge j and every leaf count is skipped.
+enum { TP_NONE = 0, TP_STAR = 1, TP_PAIR = 2, TP_SAME = 3 };
+constexpr int LV_PAT = 1 << 11;  // frame of edge M-2 summed by its pattern
+
+template <class SP>
+__device__ __forceinline__ int tail_pattern(const tm_motif_dev &md) {
+  const int M = SP::M(md);
+  if (M < 3) return TP_NONE;
+  const int j = M - 2, k = M - 1;
+  const int kj = SP::kind(md, j), kk = SP::kind(md, k);
+  const bool samepair = SP::a(md, k) == SP::a(md, j) && SP::b(md, k) == SP::b(md, j);
+  if (kj == TM_K_CLOSE) return (kk == TM_K_CLOSE && samepair) ? TP_SAME : TP_NONE;
+  // a new node and then its pair again: the leaf-by-leaf path prunes every
+  // candidate whose pair has no later edge with one load already, and the
+  // pair pattern's per-frame terms measured slower (kept as a closed form
+  // for the model and tests: TP_PAIR is not selected here)
+  if (kk == TM_K_CLOSE) return TP_NONE;
+  if (kk == kj && (kj == TM_K_NEW_DST ? SP::a(md, k) == SP::a(md, j) : SP::b(md, k) == SP::b(md, j))) return TP_STAR;
+  return TP_NONE;
+}
+
+// The per-frame terms of a pattern (mod 2^64: the pair pattern's is minus a
+// sum); p = position matched for edge j - 1, A = (p, pend) window of K slots.
+template <class SP, int MAXN>
+__device__ __forceinline__ unsigned long long tail_frame_term(const tm_graph_view &g, const tm_motif_dev &md,
+                                                              int pat, const int (&m)[MAXN], int K, int p,
+                                                              int pend, Meter *mt = nullptr) {
+  if (pat == TP_SAME) return (unsigned long long)K * (unsigned long long)(K - 1) / 2ull;
+  const int j = SP::M(md) - 2;
+  const int nm = SP::mapped(md, j);
+  const bool out = SP::kind(md, j) == TM_K_NEW_DST;
+  const int base = out ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j));
+  long long kp = K;
+  unsigned long long c2 = 0;
+#pragma unroll
+  for (int i = 0; i < MAXN; i++) {
+    if (i >= nm || (g.no_self_loops && m[i] == base)) continue;
+    const long long n = out ? edge_count(g, base, m[i], p, pend, mt) : edge_count(g, m[i], base, p, pend, mt);
+    kp -= n;
+    c2 += (unsigned long long)(n * (n - 1) / 2);
+  }
+  if (pat == TP_PAIR) return 0ull - c2;
+  return (unsigned long long)(kp * (kp - 1) / 2) + c2;
+}
+
+// sp(c) of the pattern sums: later slots of A in the window with the same
+// other end as the slot (edges of the same graph pair after it).
+template <class SP, int MAXN>
+__device__ __forceinline__ unsigned long long tail_same_pair(const tm_graph_view &g, const tm_motif_dev &md,
+                                                             const int (&m)[MAXN], int slot, int tl, int pend,
+                                                             Meter *mt = nullptr) {
+  const int j = SP::M(md) - 2;
+  const bool out = SP::kind(md, j) == TM_K_NEW_DST;
+  const int lst = out ? TM_L_OUT : TM_L_IN;
+  if (mt) mt->b += 4;
+  const int tn = g.stf[(size_t)(lst * TM_NTFIRST + TM_T_PAIR_SELF) * (size_t)g.E + slot];
+  if (tn > tl) return 0ull;
+  const int base = out ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j));
+  if (mt) mt->b += 8;
+  const int pos = out ? g.outPos[slot] : g.inPos[slot];
+  const int w = out ? g.outNbr[slot] : g.inNbr[slot];
+  return (unsigned long long)(out ? edge_count(g, base, w, pos, pend, mt) : edge_count(g, w, base, pos, pend, mt));
+}
+
+// The last edge k's injectivity term for one mapped node can be decided by
+// the candidate c of edge j itself: c is a graph edge m[a_j] -> m[b_j], and
+// when edge k's list holds edges of that same pair (or of the reversed pair)
+// towards that node, c's next edge of the pair (TM_T_PAIR_SELF) or of the
+// reversed pair (TM_T_RECIP) after it, one load, shows whether any is left
+// in the window.  Returns rule | motif node << 8, or -1.
+template <class SP>
+__device__ __forceinline__ int tail_crule(const tm_motif_dev &md) {
+  const int M = SP::M(md);
+  if (M < 3) return -1;
+  const int j = M - 2, k = M - 1;
+  const int kk = SP::kind(md, k);
+  const int aj = SP::a(md, j), bj = SP::b(md, j);
+  if (kk == TM_K_NEW_DST) {  // corrections: edges m[a_k] -> x
+    const int ak = SP::a(md, k);
+    if (ak == aj) return TM_T_PAIR_SELF | (bj << 8);
+    if (ak == bj) return TM_T_RECIP | (aj << 8);
+  } else if (kk == TM_K_NEW_SRC) {  // corrections: edges x -> m[b_k]
+    const int bk = SP::b(md, k);
+    if (bk == bj) return TM_T_PAIR_SELF | (aj << 8);
+    if (bk == aj) return TM_T_RECIP | (bj << 8);
+  }
+  return -1;
+}
+
+// Motif nodes mapped before edge j (bit i: node i) whose injectivity terms
+// against `base` (edges base -> x when out, x -> base otherwise) are 0 over
+// the whole window (p, pend) of a frame of edge j: every leaf or item of the
+// frame, whose range lies inside that window, skips them.
+template <class SP, int MAXN>
+__device__ __forceinline__ int tail_zero_mask(const tm_graph_view &g, const tm_motif_dev &md, const int (&m)[MAXN],
+                                              int base, bool out, int p, int pend, Meter *mt = nullptr) {
+  const int nm = SP::mapped(md, SP::M(md) - 2);
+  int mask = 0;
+#pragma unroll
+  for (int i = 0; i < MAXN; i++) {
+    if (i >= nm) continue;
+    if (g.no_self_loops && m[i] == base) { mask |= 1 << i; continue; }
+    const int n = out ? edge_count(g, base, m[i], p, pend, mt) : edge_count(g, m[i], base, p, pend, mt);
+    if (n == 0) mask |= 1 << i;
+  }
+  return mask;
+}
+
+// The zero mask of a frame of edge M-2 about to be pushed (0: none worth
+// computing): for a frame counted leaf by leaf, against the last edge's
+// list node when that node is mapped before edge M-2; for a swapped frame
+// (independent last edge), against A's node.  Frames of fewer than four
+// candidates skip it.
+template <class SP, int MAXN>
+__device__ __forceinline__ int tail_frame_mask(const tm_graph_view &g, const tm_motif_dev &md, const int (&m)[MAXN],
+                                               bool swapped, int K, int p, int pend, Meter *mt = nullptr) {
+  if (K < 4) return 0;
+  const int M = SP::M(md);
+  const int j = M - 2, k = M - 1;
+  const int w = SP::mapped(md, j);
+  const int kj = SP::kind(md, j), kk = SP::kind(md, k);
+  if (swapped) {
+    if (kj == TM_K_CLOSE || tail2_mode<SP>(md) != T2_INDEP) return 0;
+    const bool out = kj == TM_K_NEW_DST;
+    return tail_zero_mask<SP>(g, md, m, out ? sel(m, SP::a(md, j)) : sel(m, SP::b(md, j)), out, p, pend, mt);
+  }
+  if (kk == TM_K_CLOSE) return 0;
+  const int bn = kk == TM_K_NEW_DST ? SP::a(md, k) : SP::b(md, k);
+  if (kj != TM_K_CLOSE && bn == w) return 0;  // the list node is edge j's new node: differs per leaf
+  return tail_zero_mask<SP>(g, md, m, sel(m, bn), kk == TM_K_NEW_DST, p, pend, mt);
 }
 
 // ------------------------------------------------------------------ kernel
@@ -809,7 +1001,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       const int last = ENUM ? M - 1 : M - 2;  // last motif edge handled per candidate
       if (last == 0) {
         // count mode, M == 2: edge 1 is the final edge
-        if (ok) { cnt += leaf_count<SP>(g, md, 1, m, L, r, pend, args.scan_max, mt); st_leaves++; }
+        if (ok) { cnt += leaf_count<SP>(g, md, 1, m, L, r, pend, args.scan_max, -1, 0, mt); st_leaves++; }
         continue;
       }
       int cb = 0, ce = 0, dir = L.dir;
@@ -820,17 +1012,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       }
       // two-edge tail of a 3-edge motif: edge 1 is the second-to-last edge
       int swp = 0, t2lo = 0, t2hi = 0;
-      if (T2 && M == 3 && ok && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
+      const int pat = (T2 && M == 3) ? tail_pattern<SP>(md) : TP_NONE;
+      if (T2 && M == 3 && ok && pat != TP_NONE) {
+        cnt += tail_frame_term<SP>(g, md, pat, m, ce - cb, r, pend, mt);
+        st_leaves++;
+        if (pat == TP_SAME) ok = false; else swp = LV_PAT;
+      } else if (T2 && M == 3 && ok && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
         int blo, bhi, bdir;
         if (mt) meter.b += 4;
         tail2_range<SP>(g, md, m, L.pos[cb], pend, &blo, &bhi, &bdir, mt);
-        if (bhi - blo < ce - cb) {
+        if (2 * (bhi - blo) < ce - cb) {
           swp = LV_SWAP | (bdir ? LV_BDIR : 0);
           t2lo = cb; t2hi = ce;
           cb = blo; ce = bhi;
           ok = ce > cb;
         }
       }
+      int zm = 0;  // injectivity terms known to be 0 in this frame's window
+      if (T2 && M == 3 && ok && !(swp & LV_PAT)) zm = tail_frame_mask<SP>(g, md, m, swp != 0, ce - cb, r, pend, mt);
       unsigned long long bm = __ballot(ok);
       int n = __popcll(bm);
       if (top + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
@@ -838,8 +1037,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         r_pend[lane] = pend;  // the window of this lane's root (no frame of an older root is left)
         r_tl[lane] = tli;
         int w = top + __popcll(bm & lt_mask);
-        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = 0 | (dir << 8) | swp | (lane << 16);
-        if (T2 && swp) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
+        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = 0 | (dir << 8) | swp | (lane << 16) | (zm << 24);
+        if (T2 && (swp & LV_SWAP)) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
 #pragma unroll
         for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
         if constexpr (ENUM) f_e[w] = r;
@@ -929,17 +1128,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const bool swp = T2 && has && (lvw & LV_SWAP);
     if (T2 && swp) {
       cnt += tail2_item<SP>(g, md, m, sw_lo[myf & (WAVE - 1)], sw_hi[myf & (WAVE - 1)], dir, slot,
-                            (lvw & LV_BDIR) ? 1 : 0, mt);
+                            (lvw & LV_BDIR) ? 1 : 0, (lvw >> 24) & 0xff, mt);
+      st_leaves++;
+    }
+    // a pattern frame (last two edges on one list): the slot adds or removes
+    // the later slots of its own graph pair
+    const bool patf = T2 && has && (lvw & LV_PAT);
+    if (T2 && patf) {
+      const unsigned long long sp = tail_same_pair<SP>(g, md, m, slot, tl, pend, mt);
+      cnt += tail_pattern<SP>(md) == TP_STAR ? 0ull - sp : sp;
       st_leaves++;
     }
     bool valid = false;
-    int pos = 0, tfn = 0;
-    if (has && !swp) {
-      int node = 0;
-      // position, neighbour and the next edge's first-candidate time: three
-      // loads at the same slot of one list, issued together
-      const int lst = kind == TM_K_NEW_DST ? TM_L_OUT
-                      : (kind == TM_K_NEW_SRC ? TM_L_IN : (dir ? TM_L_INPAIR : TM_L_OUTPAIR));
+    int pos = 0, tfn = 0, node = -1;
+    // position, neighbour and the next edge's first-candidate time: three
+    // loads at the same slot of one list, issued together
+    const int lst = kind == TM_K_NEW_DST ? TM_L_OUT
+                    : (kind == TM_K_NEW_SRC ? TM_L_IN : (dir ? TM_L_INPAIR : TM_L_OUTPAIR));
+    if (has && !swp && !patf) {
       if (tmn >= 0) tfn = g.stf[(size_t)(lst * TM_NTFIRST + tmn) * (size_t)g.E + slot];
       // further rules: each later edge (k >= j + 2) at a node of this edge
       // needs that node's next edge after this position in the window (not
@@ -1013,7 +1219,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       } else {
         if (valid && (tmn < 0 || tfn <= tl)) {
           const CandList LL = cand_start<SP>(g, md, j + 1, m, pos, mt);
-          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, mt);
+          int skipx = -1;
+          const int cr = T2 ? tail_crule<SP>(md) : -1;
+          if (cr >= 0) {
+            if (mt) meter.b += 4;
+            if (g.stf[(size_t)(lst * TM_NTFIRST + (cr & 0xff)) * (size_t)g.E + slot] > tl) skipx = sel(m, cr >> 8);
+          }
+          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, skipx, (lvw >> 24) & 0xff, mt);
           st_leaves++;
         }
       }
@@ -1022,7 +1234,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       continue;
     }
     int cb = 0, ce = 0, cdir = 0;
-    int swc = 0, t2lo = 0, t2hi = 0;
+    int swc = 0, t2lo = 0, t2hi = 0, zm = 0;
     if (valid && tmn >= 0) valid = tfn <= tl;
     if (valid) {
       const CandList LL = cand_start<SP>(g, md, j + 1, m, pos, mt);
@@ -1030,25 +1242,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       cdir = LL.dir;
       ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend, mt);
       valid = ce > cb;
-      if (T2 && valid && j == M - 3 && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
+      const int pat = (T2 && j == M - 3) ? tail_pattern<SP>(md) : TP_NONE;
+      if (T2 && valid && j == M - 3 && pat != TP_NONE) {
+        cnt += tail_frame_term<SP>(g, md, pat, m, ce - cb, pos, pend, mt);
+        st_leaves++;
+        if (pat == TP_SAME) valid = false; else swc = LV_PAT;
+      } else if (T2 && valid && j == M - 3 && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
         int blo, bhi, bdir;
         if (mt) meter.b += 4;
         tail2_range<SP>(g, md, m, LL.pos[cb], pend, &blo, &bhi, &bdir, mt);
-        if (bhi - blo < ce - cb) {
+        if (2 * (bhi - blo) < ce - cb) {
           swc = LV_SWAP | (bdir ? LV_BDIR : 0);
           t2lo = cb; t2hi = ce;
           cb = blo; ce = bhi;
           valid = ce > cb;
         }
       }
+      if (T2 && valid && j == M - 3 && !(swc & LV_PAT))
+        zm = tail_frame_mask<SP>(g, md, m, swc != 0, ce - cb, pos, pend, mt);
     }
     const unsigned long long bm = __ballot(valid);
     const int n = __popcll(bm);
     if (ntop + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
     if (valid) {
       int w = ntop + __popcll(bm & lt_mask);
-      f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | swc | (ridx << 16);
-      if (T2 && swc) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
+      f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | swc | (ridx << 16) | (zm << 24);
+      if (T2 && (swc & LV_SWAP)) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
 #pragma unroll
       for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
       if constexpr (ENUM) {
@@ -1085,14 +1304,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       if (j == last) {
         if (has) {
           const CandList LL = cand_start<SP>(g, md, j + 1, m, pos, mt);
-          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, mt);
+          int skipx = -1;
+          const int cr = tail_crule<SP>(md);
+          if (cr >= 0) {
+            if (mt) meter.b += 4;
+            const int rule = cr & 0xff;  // uniform: a branch, not an indexed pointer table
+            const int tn = rule == TM_T_PAIR_SELF ? g.tfirst[TM_T_PAIR_SELF][pos] : g.tfirst[TM_T_RECIP][pos];
+            if (tn > r_tl[ridx]) skipx = sel(m, cr >> 8);
+          }
+          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, skipx, 0, mt);
           st_leaves++;
         }
         __syncthreads();
         continue;
       }
       int cb = 0, ce = 0, cdir = 0;
-      int swc = 0, t2lo = 0, t2hi = 0;
+      int swc = 0, t2lo = 0, t2hi = 0, zm = 0;
       bool valid = has;
       if (valid) {
         const CandList LL = cand_start<SP>(g, md, j + 1, m, pos, mt);
@@ -1100,25 +1327,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         cdir = LL.dir;
         ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend, mt);
         valid = ce > cb;
-        if (T2 && valid && j == M - 3 && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
+        const int pat = (T2 && j == M - 3) ? tail_pattern<SP>(md) : TP_NONE;
+        if (T2 && valid && j == M - 3 && pat != TP_NONE) {
+          cnt += tail_frame_term<SP>(g, md, pat, m, ce - cb, pos, pend, mt);
+          st_leaves++;
+          if (pat == TP_SAME) valid = false; else swc = LV_PAT;
+        } else if (T2 && valid && j == M - 3 && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
           int blo, bhi, bdir;
           if (mt) meter.b += 4;
           tail2_range<SP>(g, md, m, LL.pos[cb], pend, &blo, &bhi, &bdir, mt);
-          if (bhi - blo < ce - cb) {
+          if (2 * (bhi - blo) < ce - cb) {
             swc = LV_SWAP | (bdir ? LV_BDIR : 0);
             t2lo = cb; t2hi = ce;
             cb = blo; ce = bhi;
             valid = ce > cb;
           }
         }
+        if (T2 && valid && j == M - 3 && !(swc & LV_PAT))
+          zm = tail_frame_mask<SP>(g, md, m, swc != 0, ce - cb, pos, pend, mt);
       }
       const unsigned long long bm = __ballot(valid);
       const int n = __popcll(bm);
       if (top + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
       if (valid) {
         const int w = top + __popcll(bm & lt_mask);
-        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | swc | (ridx << 16);
-        if (T2 && swc) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
+        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | swc | (ridx << 16) | (zm << 24);
+        if (T2 && (swc & LV_SWAP)) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
 #pragma unroll
         for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
       }

@@ -84,6 +84,7 @@ typedef struct {
   // read as one 16-byte load per lane never leaves the array.
   const unsigned *rgap;    // TM_NTFIRST arrays of epad entries, one allocation
   int epad;               // entries per rgap array
+  int no_self_loops;      // 1 when no edge has u == v: corrections for a list's own node are 0
   const int *bucket;      // bucket[k] = first position with T >= tmin + (k << bshift)
   int tmin, bshift;
   long long nbuckets;

@@ -83,6 +83,8 @@ typedef struct tm_tuning {
                             second-to-last one brings, a partial match whose second-to-last edge has at
                             least this many candidates in the window is counted over the last edge's
                             candidates instead, if those are fewer; 0 = library default (32), < 0 = never */
+  int32_t fuse_classes;  /* tm_batch_count / tm_server: 1 = every query of the call in one persistent launch
+                            (its stack sized for the deepest motif), 0 = one launch per shape class */
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */

@@ -446,7 +446,7 @@ def _swap_frame(g, md, f, t2_min, frames, w):
     pos, nbr = _list_arrays(g, md, md["M"] - 2, f["dir"])
     A = (pos, nbr, f["cur"], f["end"], f["dir"])
     bpos, bnbr, blo, bhi, bdir = tail2_lists(g, md, f["m"], A, f["pend"])
-    if bhi - blo >= nA:
+    if 2 * (bhi - blo) >= nA:
         return f
     SWAPS[0] += 1
     if bhi <= blo:
@@ -473,7 +473,10 @@ def _swap_item(g, md, f, slot):
     pos, nbr = _list_arrays(g, md, M - 2, f["dir"])
     A = (pos, nbr, f["alo"], f["ahi"], f["dir"])
     kk = md["kind"][M - 1]
-    if kk == K_NEW_DST:
+    if tail2_mode(md) == T2_JOIN:
+        into_y = md["a"][M - 1] == md["mapped"][M - 2]
+        qpos, z = ((g["inPos"][slot], g["inNbr"][slot]) if into_y else (g["outPos"][slot], g["outNbr"][slot]))
+    elif kk == K_NEW_DST:
         qpos, z = g["outPos"][slot], g["outNbr"][slot]
     elif kk == K_NEW_SRC:
         qpos, z = g["inPos"][slot], g["inNbr"][slot]
@@ -960,19 +963,29 @@ def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, ste
 
 # ------------------------------------------------ two-edge tail by the shorter list
 
-def tail2_eligible(md):
-    """The last edge k = M-1 does not touch the node edge j = M-2 introduces
-    (if it introduces one): given the partial match through edge j - 1, the
-    last edge's candidate list is then fixed, and only the position order
-    (and, when both edges bring new nodes, their distinctness) ties the two."""
+T2_NONE, T2_INDEP, T2_JOIN = range(3)
+
+
+def tail2_mode(md):
+    """INDEP: the last edge k = M-1 does not touch the node edge j = M-2
+    introduces (if it introduces one): given the partial match through edge
+    j - 1, the last edge's candidate list is then fixed, and only the position
+    order (and, when both edges bring new nodes, their distinctness) ties the
+    two.  JOIN: edge j brings node w and edge k closes w with an earlier node."""
     M = md["M"]
     if M < 3:
-        return False
+        return T2_NONE
     j, k = M - 2, M - 1
     if md["kind"][j] == K_CLOSE:
-        return True
+        return T2_INDEP
     w = md["mapped"][j]  # the node edge j introduces
-    return md["a"][k] != w and md["b"][k] != w
+    if md["a"][k] != w and md["b"][k] != w:
+        return T2_INDEP
+    return T2_JOIN if md["kind"][k] == K_CLOSE else T2_NONE
+
+
+def tail2_eligible(md):
+    return tail2_mode(md) != T2_NONE
 
 
 def tail2_lists(g, md, m, A, pend):
@@ -983,6 +996,13 @@ def tail2_lists(g, md, m, A, pend):
     k = M - 1
     pa0 = A[0][A[2]]
     kind = md["kind"][k]
+    if tail2_mode(md) == T2_JOIN:
+        w = md["mapped"][k - 1]
+        into_y = md["a"][k] == w
+        y = m[md["b"][k]] if into_y else m[md["a"][k]]
+        pos, nbr, R = (g["inPos"], g["inNbr"], g["inR"]) if into_y else (g["outPos"], g["outNbr"], g["outR"])
+        lo = upper_bound(pos, R[y], R[y + 1], pa0)
+        return pos, nbr, lo, gallop_lb(pos, lo, R[y + 1], pend), 0
     if kind == K_NEW_DST:
         x = m[md["a"][k]]
         lo = upper_bound(g["outPos"], g["outR"][x], g["outR"][x + 1], pa0)
@@ -1016,6 +1036,14 @@ def tail2_item(g, md, m, A, q_pos, q_nbr):
     M = md["M"]
     j, k = M - 2, M - 1
     nm = md["mapped"][j]
+    if tail2_mode(md) == T2_JOIN:
+        if q_nbr in m[:nm]:
+            return 0
+        pos = A[0]
+        pa0 = pos[A[2]]
+        if md["kind"][j] == K_NEW_DST:
+            return edge_count(g, m[md["a"][j]], q_nbr, pa0 - 1, q_pos)
+        return edge_count(g, q_nbr, m[md["b"][j]], pa0 - 1, q_pos)
     if md["kind"][k] != K_CLOSE and q_nbr in m[:nm]:
         return 0
     pos, nbr, lo, hi, _ = A
@@ -1124,3 +1152,50 @@ def run_concurrent_launches(grids, capacity, idle_cap, work, rng, max_ticks=100_
         if not changed and len(resident) >= capacity and any(unstarted):
             return "deadlock"
     return "timeout"
+
+
+# ------------------------------------- last two edges on one list: closed forms
+
+TP_NONE, TP_STAR, TP_PAIR, TP_SAME = range(4)
+
+
+def tail_pattern(md):
+    M = md["M"]
+    if M < 3:
+        return TP_NONE
+    j, k = M - 2, M - 1
+    kj, kk = md["kind"][j], md["kind"][k]
+    samepair = md["a"][k] == md["a"][j] and md["b"][k] == md["b"][j]
+    if kj == K_CLOSE:
+        return TP_SAME if (kk == K_CLOSE and samepair) else TP_NONE
+    if kk == K_CLOSE:
+        return TP_PAIR if samepair else TP_NONE
+    if kk == kj and (md["a"][k] == md["a"][j] if kj == K_NEW_DST else md["b"][k] == md["b"][j]):
+        return TP_STAR
+    return TP_NONE
+
+
+def tail_pattern_count(g, md, m, A, p, pend):
+    """The kernel's pattern sum for one frame of edge M-2 (list A bounded by
+    the window, p = the position matched for edge M-3): the frame's closed-form
+    term plus one same-pair term per slot."""
+    pat = tail_pattern(md)
+    pos, nbr, lo, hi, _ = A
+    K = hi - lo
+    if pat == TP_SAME:
+        return K * (K - 1) // 2
+    j = md["M"] - 2
+    out = md["kind"][j] == K_NEW_DST
+    base = m[md["a"][j]] if out else m[md["b"][j]]
+    kp, c2 = K, 0
+    for x in m[:md["mapped"][j]]:
+        n = edge_count(g, base, x, p, pend) if out else edge_count(g, x, base, p, pend)
+        kp -= n
+        c2 += n * (n - 1) // 2
+    sp = 0
+    for s in range(lo, hi):
+        w = nbr[s]
+        sp += edge_count(g, base, w, pos[s], pend) if out else edge_count(g, w, base, pos[s], pend)
+    if pat == TP_PAIR:
+        return sp - c2
+    return kp * (kp - 1) // 2 + c2 - sp
