@@ -151,6 +151,24 @@ __global__ void rgap_kernel(const tm_edge *__restrict__ Eg, int E, const int *t0
   }
 }
 
+// Previous edge of the same pair, per slot of the out- and in-lists: the
+// entry before the edge's own slot in its source's out-pair index.
+__global__ void sprev_kernel(const tm_edge *__restrict__ Eg, const int *__restrict__ outPos,
+                             const int *__restrict__ inPos, const int *__restrict__ outR,
+                             const int *__restrict__ outPairNbr, const int *__restrict__ outPairPos,
+                             const int *__restrict__ outPairSelf, int E, int *__restrict__ sprev) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= E) return;
+  for (int k = 0; k < 2; k++) {
+    const int p = k ? inPos[s] : outPos[s];
+    const tm_edge e = Eg[p];
+    const int ps = outPairSelf[p];
+    int prev = -1;
+    if (ps > outR[e.u] && outPairNbr[ps - 1] == e.v) prev = outPairPos[ps - 1];
+    sprev[(size_t)k * E + s] = prev;
+  }
+}
+
 __global__ void low_word_kernel(const unsigned long long *__restrict__ k, int *__restrict__ out, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (int)(unsigned)(k[i] & 0xffffffffull);
@@ -372,7 +390,9 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
   const int nmap = maxN <= 4 ? 4 : 8;
-  const int per_frame = 3 + nmap + (enumerate ? TM_MAXE : 0);  // cur, end, level word, node map, positions
+  // cur, end, level word, node map, positions (enumeration), weight and list start (counting)
+  const bool counting = !enumerate && instr != 1;
+  const int per_frame = 3 + nmap + (enumerate ? TM_MAXE : 0) + (counting ? 2 : 0);
   // survivor compaction of the counting kernel: survivors buffered before a
   // full-width expansion (0: expanded in the step that filtered them), and
   // the survivors with which a step expands its own when nothing is buffered
@@ -388,10 +408,10 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   // frame stack (an even number of ints), the roots' window table (2 x 65
   // ints), the survivor buffer (counting), then the launch's per-query counts
   auto lds_of = [&](int flush) {
-    const int sb = flush > 0 ? (2 + nmap) * tmk::WAVE : 0;
-    const int t2 = (enumerate || instr == 1) ? 0 : 2 * tmk::WAVE;  // two-edge tail side table (counting)
-    return (frames * per_frame + 2 * tmk::TM_ROOTS + sb + t2) * (int)sizeof(int) +
-           nq * (int)sizeof(unsigned long long);
+    const int sb = flush > 0 ? (4 + nmap) * tmk::WAVE : 0;  // position, root, weight (2), node map
+    const int t2 = counting ? 2 * tmk::WAVE : 0;             // two-edge tail side table
+    const int roots = (counting ? 4 : 2) * tmk::TM_ROOTS;    // window, and the root-pair list (counting)
+    return (frames * per_frame + roots + sb + t2) * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
   };
   int wpe = 0;  // register cap of the counting kernel: 1 (none), 5 (96 VGPRs) or 6 (80 VGPRs)
   if (tune && tune->waves_per_simd_hint > 0)
@@ -893,6 +913,7 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
     ALLOC(vw.tfirst[k], e4);
   }
   ALLOC(vw.stf, e4 * 4 * TM_NTFIRST);
+  ALLOC(vw.sprev, e4 * 2);
   vw.epad = (int)(((long long)E + 255) / 256 * 256 + 256);
   ALLOC(vw.rgap, sizeof(unsigned) * (size_t)vw.epad * TM_NTFIRST);
   const int tmin = Eg[0].t, tmax = Eg[E - 1].t;
@@ -969,6 +990,9 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
       }
       TMPCHK(hipGetLastError());
     }
+    hipLaunchKernelGGL(sprev_kernel, gr, bl, 0, s, vw.Eg, vw.outPos, vw.inPos, vw.outR, vw.outPairNbr, vw.outPairPos,
+                       vw.outPairSelf, E, (int *)vw.sprev);
+    TMPCHK(hipGetLastError());
     TMPCHK(hipMemsetAsync((void *)vw.rgap, 0xff, sizeof(unsigned) * (size_t)vw.epad * TM_NTFIRST, s));
     {
       unsigned *rg = (unsigned *)vw.rgap;

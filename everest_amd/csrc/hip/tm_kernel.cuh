@@ -650,6 +650,41 @@ __device__ __forceinline__ int tail_frame_mask(const tm_graph_view &g, const tm_
   return tail_zero_mask<SP>(g, md, m, sel(m, bn), kk == TM_K_NEW_DST, p, pend, mt);
 }
 
+// ----------------------------- repeated edges of one graph pair (count mode)
+//
+// A motif edge that repeats the pair of the edge before it (smode
+// TM_S_PAIR_SELF) makes the search quadratic in the number of parallel
+// edges of that pair inside the window: every parallel edge chosen for the
+// earlier motif edge starts a subtree, and the later ones repeat the same
+// choices.  Two rewrites count each subtree once, with a multiplicity:
+//   root pair (edge 1 repeats the root's pair, M >= 4): the candidates of
+//     edge 2 after the first parallel edge p1 are expanded once, each
+//     weighted by the parallel edges before it (its rank among them);
+//   pair tail (edge M-3 brings a node w, edge M-2 repeats that pair): only
+//     the first edge of each pair in the window expands, and edge M-2's
+//     candidate at slot s of the pair list counts s - (first slot) + 1 times.
+// Weights are carried by frames (u32) and multiply the counts below them.
+constexpr int LV_RW = 1 << 12;   // level-1 frame of the root-pair rewrite: weight = rank in the root's pair list
+constexpr int LV_R1 = 1 << 13;   // frame of the pair-tail rewrite: weight = slot - first slot + 1
+
+template <class SP>
+__device__ __forceinline__ bool root_pair_ok(const tm_motif_dev &md) {
+  return SP::M(md) >= 4 && SP::kind(md, 1) == TM_K_CLOSE && SP::smode(md, 1) == TM_S_PAIR_SELF;
+}
+
+// the edge j (= M-3) whose pair edge M-2 repeats, when j brings a node; else -1
+template <class SP>
+__device__ __forceinline__ int pair_tail_edge(const tm_motif_dev &md) {
+  const int M = SP::M(md);
+  if (M < 4) return -1;
+  const int j = M - 3;
+  if (SP::kind(md, j) == TM_K_CLOSE) return -1;
+  // under the root-pair rewrite edge 1 is not one position but all of them:
+  // "first after p1" has no single p1 for edge 2
+  if (j == 2 && root_pair_ok<SP>(md)) return -1;
+  return (SP::kind(md, j + 1) == TM_K_CLOSE && SP::smode(md, j + 1) == TM_S_PAIR_SELF) ? j : -1;
+}
+
 // ------------------------------------------------------------------ kernel
 
 // LDS frame stack, structure of arrays, capacity FR frames:
@@ -696,24 +731,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   // The window of a frame (pend, tl) is its root's, kept once per root in
   // r_pend / r_tl: entries 0..63 for the roots of the quarter being
   // expanded (lane order), entry 64 for a frame taken from another wavefront.
+  // counting (not the instrumented kernel): two-edge tails, pair rewrites
+  constexpr bool T2 = !ENUM && INSTR != 1;
   int *f_cur = lds;
   int *f_end = f_cur + FR;
   int *f_lvl = f_end + FR;
   int *f_m = f_lvl + FR;
   int *f_e = f_m + MAXN * FR;
-  int *r_pend = f_e + (ENUM ? MAXE * FR : 0);
+  // (counting) the frame's weight and the position its candidate list starts after
+  int *f_w = f_e + (ENUM ? MAXE * FR : 0);
+  int *f_p0 = f_w + (T2 ? FR : 0);
+  int *r_pend = f_p0 + (T2 ? FR : 0);
   int *r_tl = r_pend + TM_ROOTS;
+  // (counting) the root's pair list [r_alo, r_ahi) of the root-pair rewrite
+  int *r_alo = r_tl + TM_ROOTS;
+  int *r_ahi = r_alo + (T2 ? TM_ROOTS : 0);
   // survivor buffer (counting, when args.sb_flush > 0): candidates of one
   // level that passed the injectivity and pruning tests, waiting to be
-  // expanded up to 64 at a time
+  // expanded up to 64 at a time, with their weights
   const bool COMPACT = !ENUM && INSTR != 1 && args.sb_flush > 0;
-  int *sb_pos = r_tl + TM_ROOTS;
+  int *sb_pos = r_ahi + (T2 ? TM_ROOTS : 0);
   int *sb_ridx = sb_pos + (COMPACT ? WAVE : 0);
-  int *sb_m = sb_ridx + (COMPACT ? WAVE : 0);
+  int *sb_wl = sb_ridx + (COMPACT ? WAVE : 0);  // weight, low and high words
+  int *sb_wh = sb_wl + (COMPACT ? WAVE : 0);
+  int *sb_m = sb_wh + (COMPACT ? WAVE : 0);
   // two-edge tail (counting): edge M-2's candidate range [sw_lo, sw_hi) of a
   // swapped frame, at the frame's stack index mod 64 (the frames of the
   // deepest level are pushed in one step, at most 64 of them)
-  constexpr bool T2 = !ENUM && INSTR != 1;
   int *sw_lo = sb_m + (COMPACT ? MAXN * WAVE : 0);
   int *sw_hi = sw_lo + (T2 ? WAVE : 0);
   // per-query counts of this wavefront, flushed once at the end
@@ -762,6 +806,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     if (f == 1) return &f_lvl[w];
     if (f == 2) return &f_cur[w];
     if (f == 3) return &f_end[w];
+    if (T2 && f == 4) return &f_p0[w];
+    if (T2 && f == 7 + TM_MAXN) return &f_w[w];
     if (f == 5) return &r_pend[(f_lvl[w] >> 16) & 0xff];
     if (f == 6) return &r_tl[(f_lvl[w] >> 16) & 0xff];
     if (f >= 7 && f < 7 + MAXN) return &f_m[(f - 7) * FR + w];
@@ -1010,6 +1056,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         ce = gallop_lb(L.pos, L.lo, L.hi, pend, mt);
         ok = ce > cb;
       }
+      // root-pair rewrite: edge 1 repeats the root's pair; expand edge 2 once
+      // from the first parallel edge, its candidates weighted by their rank
+      // among the parallel edges [a1lo, a1hi) of the window
+      int lvl0 = 0, rw = 0, p0f = r, a1lo = 0, a1hi = 0;
+      if (T2 && ok && root_pair_ok<SP>(md)) {
+        a1lo = cb;
+        a1hi = ce;
+        if (mt) meter.b += 4;
+        const int p1 = L.pos[cb];
+        const CandList L2 = cand_start<SP>(g, md, 2, m, p1, mt);
+        cb = L2.lo;
+        dir = L2.dir;
+        ce = gallop_lb(L2.pos, L2.lo, L2.hi, pend, mt);
+        ok = ce > cb;
+        lvl0 = 1;
+        rw = LV_RW;
+        p0f = p1;
+      }
       // two-edge tail of a 3-edge motif: edge 1 is the second-to-last edge
       int swp = 0, t2lo = 0, t2hi = 0;
       const int pat = (T2 && M == 3) ? tail_pattern<SP>(md) : TP_NONE;
@@ -1029,7 +1093,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
       }
       int zm = 0;  // injectivity terms known to be 0 in this frame's window
-      if (T2 && M == 3 && ok && !(swp & LV_PAT)) zm = tail_frame_mask<SP>(g, md, m, swp != 0, ce - cb, r, pend, mt);
+      if (T2 && M == 3 && ok && !(swp & LV_PAT)) zm = tail_frame_mask<SP>(g, md, m, (swp & LV_SWAP) != 0, ce - cb, r, pend, mt);
       unsigned long long bm = __ballot(ok);
       int n = __popcll(bm);
       if (top + n > FR) { if (lane == 0) atomicOr(&sched->error, 1); break; }
@@ -1037,8 +1101,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         r_pend[lane] = pend;  // the window of this lane's root (no frame of an older root is left)
         r_tl[lane] = tli;
         int w = top + __popcll(bm & lt_mask);
-        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = 0 | (dir << 8) | swp | (lane << 16) | (zm << 24);
+        f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = lvl0 | (dir << 8) | swp | rw | (lane << 16) | (zm << 24);
         if (T2 && (swp & LV_SWAP)) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
+        if (T2) {
+          f_w[w] = 1;
+          f_p0[w] = p0f;
+          r_alo[lane] = a1lo;
+          r_ahi[lane] = a1hi;
+        }
 #pragma unroll
         for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
         if constexpr (ENUM) f_e[w] = r;
@@ -1056,6 +1126,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const int last = ENUM ? M - 1 : M - 2;  // last motif edge handled per candidate
     const int Ltop = top > 0 ? uni(f_lvl[top - 1] & 0xff) : -1;
     bool flush = COMPACT && sbn > 0 && (top == 0 || Ltop != sb_level || sbn >= args.sb_flush);
+    if (T2 && !flush && top > 0) {
+      // a pattern frame of 64 slots or more on top: the wave sweeps its slots
+      // 64 at a time (one coalesced load of next same-pair times each; the
+      // rare slot whose pair repeats in the window counts those edges)
+      const int ft = top - 1;
+      const int lvt = uni(f_lvl[ft]);
+      const int c0 = uni(f_cur[ft]), c1 = uni(f_end[ft]);
+      if ((lvt & LV_PAT) && c1 - c0 >= WAVE) {
+        const int rt = (lvt >> 16) & 0xff;
+        const int tlt = uni(r_tl[rt]), pendt = uni(r_pend[rt]);
+        int m[MAXN];
+#pragma unroll
+        for (int i = 0; i < MAXN; i++) m[i] = uni(f_m[i * FR + ft]);
+        const unsigned long long fwt = (unsigned)uni(f_w[ft]);
+        unsigned long long acc = 0;
+        for (int sl = c0 + lane; sl < c1; sl += WAVE) acc += tail_same_pair<SP>(g, md, m, sl, tlt, pendt, mt);
+        acc *= fwt;
+        cnt += tail_pattern<SP>(md) == TP_STAR ? 0ull - acc : acc;
+        st_steps++;
+        st_items += (unsigned long long)(c1 - c0);
+        top = ft;
+        __syncthreads();
+        continue;
+      }
+    }
     if (!flush) {
     // ---- deal the next candidate slots of the topmost frames of one level
     // to the lanes (at most lim: room left in the survivor buffer)
@@ -1096,6 +1191,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     st_steps++;
     st_items += total;
     int slot = 0, pend = 0, dir = 0, tl = 0, ridx = 0, lvw = 0;
+    unsigned fw = 1;  // the frame's weight (counting)
     int m[MAXN];
     int e[ENUM ? MAXE : 1];
 #pragma unroll
@@ -1104,6 +1200,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       slot = f_cur[myf] + off;
       const int lv = f_lvl[myf];
       lvw = lv;
+      if (T2) fw = (unsigned)f_w[myf];
       dir = (lv >> 8) & 1;
       ridx = (lv >> 16) & 0xff;
       pend = r_pend[ridx];
@@ -1127,7 +1224,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     // edge; its matches with the earlier candidates of edge j are counted here
     const bool swp = T2 && has && (lvw & LV_SWAP);
     if (T2 && swp) {
-      cnt += tail2_item<SP>(g, md, m, sw_lo[myf & (WAVE - 1)], sw_hi[myf & (WAVE - 1)], dir, slot,
+      cnt += (unsigned long long)fw *
+             tail2_item<SP>(g, md, m, sw_lo[myf & (WAVE - 1)], sw_hi[myf & (WAVE - 1)], dir, slot,
                             (lvw & LV_BDIR) ? 1 : 0, (lvw >> 24) & 0xff, mt);
       st_leaves++;
     }
@@ -1135,7 +1233,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     // the later slots of its own graph pair
     const bool patf = T2 && has && (lvw & LV_PAT);
     if (T2 && patf) {
-      const unsigned long long sp = tail_same_pair<SP>(g, md, m, slot, tl, pend, mt);
+      const unsigned long long sp = (unsigned long long)fw * tail_same_pair<SP>(g, md, m, slot, tl, pend, mt);
       cnt += tail_pattern<SP>(md) == TP_STAR ? 0ull - sp : sp;
       st_leaves++;
     }
@@ -1170,7 +1268,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         valid = true;
       }
       if (tx > tl) valid = false;
+      // pair tail: only the first edge of its pair after the frame's start expands
+      if (T2 && valid && j == pair_tail_edge<SP>(md)) {
+        if (mt) meter.b += 4;
+        if (g.sprev[(kind == TM_K_NEW_SRC ? (size_t)g.E : 0) + slot] > f_p0[myf]) valid = false;
+      }
     }
+    // this candidate's weight: the frame's, times its rank among the root's
+    // parallel edges (root-pair rewrite) or its offset in the pair list (pair tail)
+    unsigned long long wi = fw;
+    if (T2 && valid && (lvw & LV_RW)) {
+      const int rk = lower_bound_i(g.outPairPos, r_alo[ridx], r_ahi[ridx], pos, mt) - r_alo[ridx];
+      wi = (unsigned long long)rk;
+      if (rk <= 0) valid = false;  // (its list starts after the first parallel edge)
+    }
+    if (T2 && has && (lvw & LV_R1)) wi = (unsigned long long)fw * (unsigned long long)(slot - sw_lo[myf & (WAVE - 1)] + 1);
     // survivors are buffered while they are few and more of this level
     // follows; otherwise (nothing buffered, and sb_inline survivors or the
     // level's last candidates in hand) they are expanded in this step
@@ -1186,6 +1298,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           const int w = sbn + __popcll(bm & lt_mask);
           sb_pos[w] = pos;
           sb_ridx[w] = ridx;
+          sb_wl[w] = (int)(unsigned)(wi & 0xffffffffull);
+          sb_wh[w] = (int)(unsigned)(wi >> 32);
 #pragma unroll
           for (int i = 0; i < MAXN; i++) sb_m[i * WAVE + w] = m[i];
         }
@@ -1225,7 +1339,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             if (mt) meter.b += 4;
             if (g.stf[(size_t)(lst * TM_NTFIRST + (cr & 0xff)) * (size_t)g.E + slot] > tl) skipx = sel(m, cr >> 8);
           }
-          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, skipx, (lvw >> 24) & 0xff, mt);
+          cnt += wi * leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, skipx, (lvw >> 24) & 0xff, mt);
           st_leaves++;
         }
       }
@@ -1243,8 +1357,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend, mt);
       valid = ce > cb;
       const int pat = (T2 && j == M - 3) ? tail_pattern<SP>(md) : TP_NONE;
-      if (T2 && valid && j == M - 3 && pat != TP_NONE) {
-        cnt += tail_frame_term<SP>(g, md, pat, m, ce - cb, pos, pend, mt);
+      if (T2 && valid && j == M - 3 && j == pair_tail_edge<SP>(md)) {
+        swc = LV_R1;  // pair tail: the pair list after this first edge, weighted by offset
+        t2lo = cb;
+      } else if (T2 && valid && j == M - 3 && pat != TP_NONE) {
+        cnt += wi * tail_frame_term<SP>(g, md, pat, m, ce - cb, pos, pend, mt);
         st_leaves++;
         if (pat == TP_SAME) valid = false; else swc = LV_PAT;
       } else if (T2 && valid && j == M - 3 && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
@@ -1259,7 +1376,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
       }
       if (T2 && valid && j == M - 3 && !(swc & LV_PAT))
-        zm = tail_frame_mask<SP>(g, md, m, swc != 0, ce - cb, pos, pend, mt);
+        zm = tail_frame_mask<SP>(g, md, m, (swc & LV_SWAP) != 0, ce - cb, pos, pend, mt);
     }
     const unsigned long long bm = __ballot(valid);
     const int n = __popcll(bm);
@@ -1267,7 +1384,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     if (valid) {
       int w = ntop + __popcll(bm & lt_mask);
       f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | swc | (ridx << 16) | (zm << 24);
-      if (T2 && (swc & LV_SWAP)) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
+      if (T2 && (swc & (LV_SWAP | LV_R1))) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
+      if (T2) {
+        f_w[w] = (int)(unsigned)wi;
+        f_p0[w] = pos;
+      }
 #pragma unroll
       for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
       if constexpr (ENUM) {
@@ -1289,12 +1410,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       const int j = sb_level + 1;  // motif edge of the buffered candidates
       const bool has = lane < sbn;
       int pos = 0, ridx = 0, pend = 0;
+      unsigned long long wi = 1;
       int m[MAXN];
 #pragma unroll
       for (int i = 0; i < MAXN; i++) m[i] = 0;
       if (has) {
         pos = sb_pos[lane];
         ridx = sb_ridx[lane];
+        wi = ((unsigned long long)(unsigned)sb_wh[lane] << 32) | (unsigned)sb_wl[lane];
         pend = r_pend[ridx];
 #pragma unroll
         for (int i = 0; i < MAXN; i++) m[i] = sb_m[i * WAVE + lane];
@@ -1312,7 +1435,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             const int tn = rule == TM_T_PAIR_SELF ? g.tfirst[TM_T_PAIR_SELF][pos] : g.tfirst[TM_T_RECIP][pos];
             if (tn > r_tl[ridx]) skipx = sel(m, cr >> 8);
           }
-          cnt += leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, skipx, 0, mt);
+          cnt += wi * leaf_count<SP>(g, md, j + 1, m, LL, pos, pend, args.scan_max, skipx, 0, mt);
           st_leaves++;
         }
         __syncthreads();
@@ -1328,8 +1451,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         ce = gallop_lb(LL.pos, LL.lo, LL.hi, pend, mt);
         valid = ce > cb;
         const int pat = (T2 && j == M - 3) ? tail_pattern<SP>(md) : TP_NONE;
-        if (T2 && valid && j == M - 3 && pat != TP_NONE) {
-          cnt += tail_frame_term<SP>(g, md, pat, m, ce - cb, pos, pend, mt);
+        if (T2 && valid && j == M - 3 && j == pair_tail_edge<SP>(md)) {
+          swc = LV_R1;
+          t2lo = cb;
+        } else if (T2 && valid && j == M - 3 && pat != TP_NONE) {
+          cnt += wi * tail_frame_term<SP>(g, md, pat, m, ce - cb, pos, pend, mt);
           st_leaves++;
           if (pat == TP_SAME) valid = false; else swc = LV_PAT;
         } else if (T2 && valid && j == M - 3 && ce - cb >= args.t2_min && tail2_ok<SP>(md)) {
@@ -1344,7 +1470,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           }
         }
         if (T2 && valid && j == M - 3 && !(swc & LV_PAT))
-          zm = tail_frame_mask<SP>(g, md, m, swc != 0, ce - cb, pos, pend, mt);
+          zm = tail_frame_mask<SP>(g, md, m, (swc & LV_SWAP) != 0, ce - cb, pos, pend, mt);
       }
       const unsigned long long bm = __ballot(valid);
       const int n = __popcll(bm);
@@ -1352,7 +1478,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       if (valid) {
         const int w = top + __popcll(bm & lt_mask);
         f_cur[w] = cb; f_end[w] = ce; f_lvl[w] = j | (cdir << 8) | swc | (ridx << 16) | (zm << 24);
-        if (T2 && (swc & LV_SWAP)) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
+        if (T2 && (swc & (LV_SWAP | LV_R1))) { sw_lo[w & (WAVE - 1)] = t2lo; sw_hi[w & (WAVE - 1)] = t2hi; }
+        if (T2) {
+          f_w[w] = (int)(unsigned)wi;
+          f_p0[w] = pos;
+        }
 #pragma unroll
         for (int i = 0; i < MAXN; i++) f_m[i * FR + w] = m[i];
       }
@@ -1380,7 +1510,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       for (int d = 0; d < want; d++) {
         // split the shallowest frame with at least 2 * steal_min candidates left
         // (a swapped frame keeps its edge M-2 range in this wavefront's side table: not donated)
-        const bool can = lane < top && !(f_lvl[lane] & LV_SWAP) && (f_end[lane] - f_cur[lane] >= 2 * args.steal_min);
+        const bool can = lane < top && !(f_lvl[lane] & (LV_SWAP | LV_RW | LV_R1)) &&
+                         (f_end[lane] - f_cur[lane] >= 2 * args.steal_min);
         const unsigned long long cm = __ballot(can);
         if (cm == 0ull) break;
         const int b = __ffsll((long long)cm) - 1;

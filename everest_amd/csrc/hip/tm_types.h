@@ -84,6 +84,9 @@ typedef struct {
   // read as one 16-byte load per lane never leaves the array.
   const unsigned *rgap;    // TM_NTFIRST arrays of epad entries, one allocation
   int epad;               // entries per rgap array
+  // Per slot of the out-lists (entries 0..E-1) and of the in-lists (E..2E-1):
+  // the position of the previous edge of the same graph pair, or -1
+  const int *sprev;
   int no_self_loops;      // 1 when no edge has u == v: corrections for a list's own node are 0
   const int *bucket;      // bucket[k] = first position with T >= tmin + (k << bshift)
   int tmin, bshift;

@@ -382,3 +382,39 @@ def test_two_edge_tail_and_meter(small_cases):
         assert c == want and (b > 0 or want == 0), (i, c, b)
         assert dg.batch_count([(mot, delta)] * 3, tune=abi.tuning(tail_swap=1)) == [want] * 3
         dg.close()
+
+
+def test_repeated_pair_rewrites():
+    """Motifs that repeat a pair (the root's pair at edge 1; a new node's pair
+    at edge M-2), on graphs dense in parallel edges: the counting kernel's
+    weighted rewrites, with and without survivor compaction, forced
+    donation, the two-edge tail and the batch kernel, equal the oracle."""
+    abi = _abi()
+    rng = np.random.default_rng(17)
+    shapes = [[(0, 1), (0, 1), (2, 1), (0, 1), (2, 3)], [(0, 1), (0, 2), (0, 2), (3, 2)],
+              [(0, 1), (0, 1), (0, 2), (0, 2)], [(0, 1), (0, 1), (0, 2), (0, 2), (3, 2)],
+              [(0, 1), (2, 1), (2, 1), (1, 3)], [(0, 1), (0, 1), (1, 2), (1, 2), (2, 0)],
+              [(0, 1), (0, 1), (0, 1), (0, 2)], [(0, 1), (1, 0), (1, 2), (1, 2), (2, 3), (3, 2)],
+              [(0, 1), (0, 2), (3, 2), (0, 1)], [(0, 1), (0, 2), (0, 2), (3, 2), (3, 0)]]
+    tunes = [None, abi.tuning(sb_flush=-1), abi.tuning(tail_swap=1),
+             abi.tuning(steal_after_us=1, steal_every=1, steal_min=1)]
+    for case in range(40):
+        # Time span a tenth to a third of the edge count: dense in parallel
+        # edges (counts up to ~1e8) while the oracle stays near a second for
+        # all 40 cases; denser settings put single cases at hours of CPU.
+        E = int(rng.integers(200, 1500))
+        N = int(rng.integers(3, 8))
+        U, V, T = random_graph(rng, E, N, int(rng.integers(E // 10, E // 3)))
+        edges = shapes[case % len(shapes)]
+        mot = [(a, b, i) for i, (a, b) in enumerate(edges)]
+        delta = int(rng.integers(0, 60))
+        g = prep.preprocess(U, V, T)
+        say("repeated pairs case", case, "E", E, edges, "delta", delta)
+        want = oracle.count(g, mot, delta, nthreads=4)
+        dg = abi.DeviceGraph.from_dict(g)
+        for t in tunes:
+            assert dg.count(mot, delta, tune=t) == want, (case, edges, delta)
+        assert dg.batch_count([(mot, delta), (mot, delta + 1)])[0] == want
+        c, _ = dg.count_bytes(mot, delta)
+        assert c == want
+        dg.close()
