@@ -72,6 +72,11 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300, help="requests per rate (0 = no serving run)")
     ap.add_argument("--serve-enum-frac", type=float, default=0.2, help="share of enumerate requests")
     ap.add_argument("--serve-enum-cap", type=int, default=1000000, help="row cap of enumerate requests")
+    ap.add_argument("--budget-s", type=float, default=520.0,
+                    help="wall-clock budget of the whole run: serving and the CPU baseline get what the plain "
+                         "pass, the metered pass and the K+W batch steps leave")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="plain pass, then the serving sweep with --serve-requests per rate, no time budget")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also copy this many bytes device-to-device (HBM counter scale for profiles)")
     args = ap.parse_args()
@@ -88,6 +93,7 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(3)
 
+    t_start = time.time()
     t0 = time.time()
     U, V, T = graphs.make(args.graph)
     el = core.data.EdgeListLoader.fromArrays(args.graph, U, V, T)
@@ -137,6 +143,17 @@ def main():
     if args.plain_only:
         log("plain pass only (--plain-only)")
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.serve_only:
+        rates = [float(x) for x in args.serve_rates.split(",") if x.strip()]
+        sweep = [serve(dg, motifs, counts, rate, args, args.serve_requests) for rate in rates]
+        line = json.dumps({"serving_sweep": sweep, "plain_pass_qps": round(plain_qps, 3),
+                           "plain_mean_service_ms": round(plain_total_s * 1e3 / nq, 3),
+                           "workload": "%d queries seed %d on %s" % (nq, args.seed, args.graph)})
+        print(line, flush=True)
+        if args.json_out:
+            with open(args.json_out, "w") as f:
+                f.write(line + "\n")
         return
 
     # ---- roofline of the dominant single-query counting kernel (largest total
@@ -188,16 +205,22 @@ def main():
     batch_qps = nq / (ms_per_step / 1e3)
     matches = float(sum(counts))
 
-    # ---- serving sweep
+    # ---- serving sweep and CPU baseline in what the budget leaves: each rate
+    # gets an equal share of 70% of it, sized by the plain pass's mean
+    # service time (the plain loop and the server each serve the stream once)
+    left = args.budget_s - (time.time() - t_start)
+    log("%.0fs of the %.0fs budget left for serving and the CPU baseline" % (left, args.budget_s))
     serving = None
-    if args.serve_requests > 0:
-        rates = [float(x) for x in args.serve_rates.split(",") if x.strip()]
-        serving = [serve(dg, motifs, counts, rate, args) for rate in rates]
-
-    # ---- CPU baseline
+    rates = [float(x) for x in args.serve_rates.split(",") if x.strip()]
+    if args.serve_requests > 0 and rates and left > 30:
+        mean_s = plain_total_s / nq
+        per_rate = 0.7 * left / len(rates)
+        n_req = int(max(20, min(args.serve_requests, per_rate / (2.5 * mean_s))))
+        serving = [serve(dg, motifs, counts, rate, args, n_req) for rate in rates]
+    left = args.budget_s - (time.time() - t_start)
     cpu = None
-    if args.cpu_seconds > 0:
-        cpu = cpu_baseline(hg, arrs, motifs, dg, args.cpu_seconds)
+    if args.cpu_seconds > 0 and left > 10:
+        cpu = cpu_baseline(hg, arrs, motifs, dg, min(args.cpu_seconds, left / 4))
 
     res = {
         "metric": "queries/s + p99 latency (batched/served vs plain) on 1 GPU; bit-exact counts",
@@ -251,7 +274,7 @@ def main():
             f.write(line + "\n")
 
 
-def serve(dg, motifs, counts, rate, args):
+def serve(dg, motifs, counts, rate, args, n):
     """One serving run at `rate` requests/s: requests drawn from the query set
     (seeded), a share of them enumerate requests, arriving as a Poisson
     stream.  Served: a client thread submits to the server at the arrival
@@ -264,7 +287,6 @@ def serve(dg, motifs, counts, rate, args):
     from everest_amd import abi
 
     rng = np.random.default_rng(args.seed * 1000 + int(rate))
-    n = args.serve_requests
     pick = [int(x) for x in rng.integers(0, len(motifs), n)]
     enum = rng.random(n) < args.serve_enum_frac
     gaps = rng.exponential(1.0 / rate, n)
@@ -371,8 +393,10 @@ def cpu_baseline(hg, arrs, motifs, dg, budget_s):
     """The C oracle (the reference's search restated, multithreaded) on four
     seeded root windows of every query; per query, the time per root over
     its windows times its root count, summed over the set -> queries/s; the
-    median of 3 repetitions, with the spread of the per-window estimates.
-    Every window's count is checked against the GPU on the same roots."""
+    median of 3 repetitions (the same windows), with the spread of the
+    per-window estimates.  Window sizes are set by a sizing pass so that one
+    repetition takes about budget_s of wall time.  Every window's count is
+    checked against the GPU on the same roots."""
     import numpy as np
     from oracle import oracle
 
@@ -381,18 +405,38 @@ def cpu_baseline(hg, arrs, motifs, dg, budget_s):
     # the host cores this process may use (a GPU box grants 16 per GPU)
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
     E = hg.numEdges
-    span, nwin = 1 << 13, 4
-    reps = []
-    spread = []
+    nwin, cal = 4, 4
+    target = budget_s / (len(motifs) * nwin)
+    rng = np.random.default_rng(17)
+    # sizing pass: each window grows from a seeded start in chunks of 4, 4, 8,
+    # 16, ... roots until it has taken `target` seconds (or reaches 8192
+    # roots); per-root cost is heavy-tailed (hub roots), so a window is sized
+    # by its own time, not by a sample elsewhere
+    windows = []
+    t = time.perf_counter()
+    for i, (m, d) in enumerate(motifs):
+        wins = []
+        for b in rng.integers(0, E - 8192, nwin):
+            beg = cur = int(b)
+            chunk, spent = cal, 0.0
+            while cur - beg < 8192 and (cur == beg or spent < target):
+                t1 = time.perf_counter()
+                oracle.count(g, m, d, cur, cur + chunk, nthreads=threads)
+                spent += time.perf_counter() - t1
+                cur += chunk
+                chunk = min(8192 - (cur - beg), chunk * 2 if cur - beg >= 2 * cal else chunk)
+            wins.append((beg, cur - beg))
+        windows.append(wins)
+        if (i + 1) % 64 == 0:
+            log("cpu baseline: sized the windows of %d/%d queries, %.1fs" % (i + 1, len(motifs), time.perf_counter() - t))
+    reps, spread = [], []
     for rep in range(3):
-        rng = np.random.default_rng(17)  # the same windows every repetition
         est_s, used = 0.0, 0.0
         lo_hi = [0.0, 0.0]
         for i, (m, d) in enumerate(motifs):
             roots = E - len(m) + 1
             per_root = []
-            for w in range(nwin):
-                beg = int(rng.integers(0, E - span))
+            for beg, span in windows[i]:
                 t = time.perf_counter()
                 c = oracle.count(g, m, d, beg, beg + span, nthreads=threads)
                 dt = time.perf_counter() - t
@@ -403,19 +447,24 @@ def cpu_baseline(hg, arrs, motifs, dg, budget_s):
             est_s += float(np.mean(per_root)) * roots
             lo_hi[0] += min(per_root) * roots
             lo_hi[1] += max(per_root) * roots
+            if (i + 1) % 64 == 0:
+                log("cpu baseline repetition %d: %d/%d queries, %.1fs" % (rep, i + 1, len(motifs), used))
         reps.append(len(motifs) / est_s)
         spread.append((len(motifs) / lo_hi[1], len(motifs) / lo_hi[0]))
-        log("cpu baseline repetition %d: %.6f queries/s (windows %.1fs of CPU time)" % (rep, reps[-1], used))
+        log("cpu baseline repetition %d: %.6f queries/s (windows %.1fs of wall time)" % (rep, reps[-1], used))
         if used > budget_s * 3:
             break
     order = np.argsort(reps)
     med = int(order[len(order) // 2])
+    spans = [w[0][1] for w in windows]
     return {"value": round(reps[med], 6), "unit": "queries/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(), "repetitions": [round(x, 6) for x in reps],
             "window_spread_qps": [round(spread[med][0], 6), round(spread[med][1], 6)],
-            "sample": "C oracle (the reference's search restated), all %d queries x %d seeded windows of %d roots, "
-                      "time per root extrapolated to each query's %d roots; median of %d repetitions on %d "
-                      "threads; GPU bit-exact on every window" % (len(motifs), nwin, span, E, len(reps), threads)}
+            "sample": "C oracle (the reference's search restated), all %d queries x %d seeded windows of %d-%d "
+                      "roots (median %d; each grown from %d roots until it took its share of the time), time per root "
+                      "extrapolated to each query's %d roots; median of %d repetitions on %d threads; GPU "
+                      "bit-exact on every window" % (len(motifs), nwin, min(spans), max(spans),
+                                                      int(np.median(spans)), cal, E, len(reps), threads)}
 
 
 if __name__ == "__main__":

@@ -38,7 +38,8 @@ class Tuning(ctypes.Structure):
                 ("steal_every", ctypes.c_int32), ("steal_max", ctypes.c_int32),
                 ("steal_min", ctypes.c_int32), ("steal_after_us", ctypes.c_int32),
                 ("waves_per_simd_hint", ctypes.c_int32), ("sb_flush", ctypes.c_int32),
-                ("sb_inline", ctypes.c_int32), ("tail_swap", ctypes.c_int32), ("fuse_classes", ctypes.c_int32)]
+                ("sb_inline", ctypes.c_int32), ("tail_swap", ctypes.c_int32), ("fuse_classes", ctypes.c_int32),
+                ("sweep", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -166,11 +167,12 @@ class DeviceGraph:
         return dict(device_bytes=b.value, num_edges=e.value, num_vertices=v.value, device=d.value)
 
     def last_stats(self):
-        st = np.zeros(16, dtype=np.uint64)
-        check(lib().tm_last_stats(self.handle, _p(st), 16))
+        st = np.zeros(17, dtype=np.uint64)
+        check(lib().tm_last_stats(self.handle, _p(st), 17))
         return dict(steps=int(st[0]), items=int(st[1]), roots=int(st[2]), leaves=int(st[3]), pushes=int(st[4]),
                     donated=int(st[8]), taken=int(st[9]), capexit=int(st[10]), work_ticks=int(st[11]),
-                    grid=int(st[12]), claim_blocks=int(st[13]), lds=int(st[14]), wpe=int(st[15]))
+                    grid=int(st[12]), claim_blocks=int(st[13]), lds=int(st[14]), wpe=int(st[15]),
+                    sweep=int(st[16]))
 
     def last_kernel_ms(self):
         ms = ctypes.c_double()

@@ -360,6 +360,19 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
   return TM_OK;
 }
 
+// The window sweep costs a fixed O(E log E) per query; the mining kernel
+// costs about the windows' slots summed over the roots.  Their mean is
+// estimated as sum_x deg_root(x) * deg_window(x) * min(1, delta / time span)
+// / E over the anchor nodes x; measured on a 64M-edge power-law graph the
+// two cost the same near 8 slots per root (mean windows of 0.02 to 3 slots:
+// mining 1-30 ms, sweep 15-60 ms; 22 to 350 slots: mining 0.1-3.3 s).
+static bool sweep_pays(const tm_graph *g, const SweepItem &it, int delta) {
+  const bool root_out = it.anchor == 0, win_out = it.out != 0;
+  const double d2 = root_out == win_out ? g->deg2[root_out ? 0 : 1] : g->deg2[2];
+  const double frac = g->tspan > 0 ? std::min(1.0, (double)delta / g->tspan) : 1.0;
+  return d2 * frac >= 8.0 * (double)g->E;
+}
+
 // Kernel variants (tm_variants.h): node map of 4 or 8, count / enumerate /
 // instrumented count; counting also comes with register caps of 96 and 80
 // VGPRs, and in a single-query and a multi-query (batch) form.
@@ -378,6 +391,27 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
                tm_launch_fn fn, void *user, Plan *pl) {
   if (nq > 1 && (enumerate || instr || fn || !multi)) return fail(TM_ERR_INTERNAL, "several queries need the batch kernel");
   Scratch &s = sl.s;
+  // counting queries of a window-sweep shape go to that kernel (unless the
+  // caller's tuning keeps them on the mining kernel); the mining launch
+  // sees them with an empty root range
+  pl->sweep.clear();
+  std::vector<char> swept(nq, 0);
+  if (!enumerate && instr == 0 && !fn && !(tune && tune->sweep < 0)) {
+    for (int i = 0; i < nq; i++) {
+      SweepItem it;
+      if (sweep_shape(s.h_queries[q0 + i].m, &it) &&
+          ((tune && tune->sweep > 0) || sweep_pays(g, it, s.h_queries[q0 + i].delta))) {
+        it.q = i;
+        pl->sweep.push_back(it);
+        swept[i] = 1;
+      }
+    }
+  }
+  auto range_of = [&](int i, int *b, int *e) {
+    const tm_query_dev &qd = s.h_queries[q0 + i];
+    *b = qd.root_beg;
+    *e = swept[i] ? qd.root_beg : qd.root_end;
+  };
   int maxM = 2, maxN = 2;
   long long span = 0;  // roots over all queries (aligned down to 4 per query)
   for (int i = 0; i < nq; i++) {
@@ -385,7 +419,9 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
     if (qd.m.tmode[1] < 0 || qd.m.tmode[1] >= TM_NTFIRST) return fail(TM_ERR_INTERNAL, "edge 1 has no start rule");
     maxM = std::max(maxM, qd.m.M);
     maxN = std::max(maxN, qd.m.n);
-    if (qd.root_end > qd.root_beg) span += qd.root_end - (qd.root_beg & ~3);
+    int rb, re;
+    range_of(i, &rb, &re);
+    if (re > rb) span += re - (rb & ~3);
   }
   const int lvls = enumerate ? std::max(1, maxM - 1) : std::max(1, maxM - 2);
   const int frames = 64 * lvls;
@@ -454,15 +490,17 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   for (int i = 0; i < nq; i++) {
     const tm_query_dev &qd = s.h_queries[q0 + i];
     int *qt = &s.h_qtab[4 * (size_t)(q0 + i)];
+    int rb, re;
+    range_of(i, &rb, &re);
     qt[0] = qd.delta;
-    qt[1] = qd.root_beg;
-    qt[2] = qd.root_end;
+    qt[1] = rb;
+    qt[2] = re;
     // start rule of edge 1 and, outside the instrumented kernel (which
     // follows the reference's search tree), the root's second rule
     qt[3] = qd.m.tmode[1] | (instr == 1 ? 0 : qd.m.xmask[0] << TM_QT_XRULE_SHIFT);
     s.h_prefix[q0 + i] = total_blocks;
-    if (qd.root_end > qd.root_beg) {
-      const long long nb = (qd.root_end - (qd.root_beg & ~3) + 255) / 256;
+    if (re > rb) {
+      const long long nb = (re - (rb & ~3) + 255) / 256;
       total_blocks += nb;
       max_blocks = std::max(max_blocks, nb);
     }
@@ -521,6 +559,7 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   sl.last_grid = grid;
   sl.last_lds = lds;
   sl.last_wpe = fn ? 0 : wpe;
+  sl.last_sweep = (int)pl->sweep.size();
   pl->a = a;
   pl->d_counts = s.d_counts + q0;
   pl->d_stops = s.d_stops + q0;
@@ -570,6 +609,14 @@ tm_status enqueue(tm_graph *g, Slot &sl, const Plan &pl, bool timing, bool count
     if ((st = dispatch<false>(sl, pl.nmap, pl.enumerate, pl.instr, pl.wpe, pl.multi, pl.lds, &blocks, &pl.a,
                               pl.grid)) != TM_OK)
       return st;
+  }
+  // the window sweep after the mining kernel on the same stream: it adds
+  // its queries' counts to the slots the reset kernel cleared
+  for (size_t i = 0; i < pl.sweep.size(); i += TM_SWEEP_MAXQ) {
+    SweepBatch b;
+    b.n = (int)std::min<size_t>(TM_SWEEP_MAXQ, pl.sweep.size() - i);
+    for (int k = 0; k < b.n; k++) b.item[k] = pl.sweep[i + k];
+    if ((st = sweep_enqueue(g->view, pl.a.queries, b, sl.stream)) != TM_OK) return st;
   }
   if (timing) HIPTRY(hipEventRecord(sl.ev1, sl.stream));
   hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, sl.stream, pl.a.wstats, pl.grid, pl.a.sched);
@@ -669,7 +716,8 @@ tm_status collect(tm_graph *g, Slot &sl, const tm_sched &hs) {
   // the statistics land in the graph's slot of that index (sl may be a view
   // of it with other device buffers)
   Slot &dst = g->slot[sl.index];
-  memcpy(dst.last_stats, hs.stats, sizeof(dst.last_stats));
+  memcpy(dst.last_stats, hs.stats, sizeof(hs.stats));
+  dst.last_stats[TM_STAT_SWEEP] = (unsigned long long)sl.last_sweep;
   dst.last_stats[TM_STAT_GRID] = (unsigned long long)sl.last_grid;
   dst.last_stats[TM_STAT_CLAIM] = (unsigned long long)sl.last_claim;
   dst.last_stats[TM_STAT_LDS] = (unsigned long long)sl.last_lds;
@@ -733,10 +781,15 @@ static tm_status enum_all(tm_graph *g, Slot &sl, int M, const int32_t *src, cons
 tm_status enum_cut(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta, int32_t beg,
                    int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows, uint64_t *count,
                    int64_t *rows_written) {
-  (void)tune;  // the cut is found with the counting kernel's own launch parameters
+  // the cut is found with the counting kernel's own launch parameters (its
+  // class's), and the caller's choice of counting path
   *rows_written = 0;
   if (cap == 0) { *count = 0; return TM_OK; }
-  const tm_tuning *ct = class_tuning_of(g, M, src, dst);
+  tm_tuning cut_tune;
+  memset(&cut_tune, 0, sizeof(cut_tune));
+  if (const tm_tuning *cls = class_tuning_of(g, M, src, dst)) cut_tune = *cls;
+  if (tune && tune->sweep) cut_tune.sweep = tune->sweep;
+  const tm_tuning *ct = &cut_tune;
   // Find the root r* at which the cumulative count over [beg, r] first
   // reaches the cap: ranges of growing width after beg, then the range that
   // holds the crossing cut into TM_CUT_PARTS parts, down to one root.
@@ -859,6 +912,15 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   g->device = device;
   g->E = E;
   g->V = V;
+  // degree products per node (root list x window list) and the time span:
+  // the window sweep's break-even estimate
+  for (int x = 0; x < V; x++) {
+    const double o = hg->outEdgesR[x + 1] - hg->outEdgesR[x], i = hg->inEdgesR[x + 1] - hg->inEdgesR[x];
+    g->deg2[0] += o * o;
+    g->deg2[1] += i * i;
+    g->deg2[2] += o * i;
+  }
+  g->tspan = (double)Eg[E - 1].t - (double)Eg[0].t;
   hipDeviceProp_t prop;
   hipError_t he = hipGetDeviceProperties(&prop, device);
   if (he != hipSuccess) { delete g; return fail(TM_ERR_HIP, hipGetErrorString(he)); }
@@ -1097,7 +1159,7 @@ tm_status tm_device_free(tm_graph *g, void *device_ptr) {
 
 tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n) {
   if (!g || !stats || n < 0) return fail(TM_ERR_ARG, "bad argument");
-  for (int i = 0; i < n && i < 16; i++) stats[i] = g->slot[g->last_slot].last_stats[i];
+  for (int i = 0; i < n && i < TM_NSTATS_HOST; i++) stats[i] = g->slot[g->last_slot].last_stats[i];
   return TM_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "tm_abi.h"
+#include "tm_sweep.h"
 #include "tm_types.h"
 
 #define TM_LAUNCH_MAXQ 512  // queries per launch (their table and counts live in LDS)
@@ -46,8 +47,8 @@ struct Slot {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the mining kernel
   Scratch s;
   float last_kernel_ms = 0.f;
-  int last_claim = 0, last_grid = 0, last_lds = 0, last_wpe = 0;
-  unsigned long long last_stats[16] = {};
+  int last_claim = 0, last_grid = 0, last_lds = 0, last_wpe = 0, last_sweep = 0;
+  unsigned long long last_stats[TM_NSTATS_HOST] = {};
 };
 
 tm_status set_error(tm_status s, const std::string &msg);
@@ -59,6 +60,8 @@ tm_status build_motif(int M, const int32_t *src, const int32_t *dst, tm_motif_de
 struct tm_graph {
   int device = 0;
   int E = 0, V = 0;
+  double deg2[3] = {};  // sums over nodes of out*out, in*in and out*in degrees
+  double tspan = 0;     // last edge time minus first
   int num_cus = 0;
   long long wall_rate_khz = 100000;  // wall_clock64 ticks per millisecond
   std::vector<void *> allocs;
@@ -93,6 +96,7 @@ struct Plan {
   int instr = 0;  // 1: reference work, 2: metered bytes
   tm_launch_fn fn = nullptr;
   void *user = nullptr;
+  std::vector<SweepItem> sweep;  // queries counted by the window-sweep kernel (empty root range in the mining launch)
 };
 // Plan a launch of staged queries [q0, q0 + nq) of the slot's staging
 // (occupancy, variant, claim size, query table) and upload its tables.

@@ -162,6 +162,8 @@ enum {
   TM_STAT_WPE = 15     //   register-cap variant (1: none, 6: 80 VGPRs, 0: caller's kernel)
 };
 #define TM_NSTATS 16  // per-wavefront counter slot (one 128-byte line)
+#define TM_STAT_SWEEP 16      // host only: queries of the launch counted by the window-sweep kernel
+#define TM_NSTATS_HOST 17
 
 typedef struct {
   tm_graph_view g;

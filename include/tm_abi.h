@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define TM_ABI_VERSION 4
+#define TM_ABI_VERSION 5
 #define TM_MAX_MOTIF_EDGES 8
 #define TM_MAX_MOTIF_NODES 8
 
@@ -85,6 +85,11 @@ typedef struct tm_tuning {
                             candidates instead, if those are fewer; 0 = library default (32), < 0 = never */
   int32_t fuse_classes;  /* tm_batch_count / tm_server: 1 = every query of the call in one persistent launch
                             (its stack sized for the deepest motif), 0 = one launch per shape class */
+  int32_t sweep;         /* counting, 3-edge motifs whose last two edges take candidates from one list of a
+                            root node (a star of two new nodes, or a new node's pair twice): > 0 = counted by
+                            the window-sweep kernel (O(E log E) per query whatever delta), < 0 = by the
+                            mining kernel like every other motif, 0 = the sweep when the roots' windows
+                            are estimated (node degrees x delta / time span) at 8 or more slots on average */
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */
@@ -155,7 +160,9 @@ tm_status tm_device_free(tm_graph *g, void *device_ptr);
  * wavefronts that left early, [11] wall-clock ticks (100 MHz) wavefronts held
  * work, summed, and the launch shape: [12]
  * wavefronts in the grid, [13] blocks per claim, [14] LDS bytes per
- * wavefront, [15] register-cap variant.  Up to 16 entries.
+ * wavefront, [15] register-cap variant, [16] queries of the launch counted
+ * by the window-sweep kernel (their work is not in [0..11]).  Up to 17
+ * entries.
  * No reference counterpart (instrumentation).
  */
 tm_status tm_last_stats(const tm_graph *g, uint64_t *stats, int32_t n);

@@ -1199,3 +1199,113 @@ def tail_pattern_count(g, md, m, A, p, pend):
     if pat == TP_PAIR:
         return sp - c2
     return kp * (kp - 1) // 2 + c2 - sp
+
+
+# ------------------------------------------------------------------ window sweep
+# 3-edge motifs whose last two edges take candidates from one list of a root
+# node (the anchor): a star (two new nodes on the anchor's out- or in-list)
+# or a new node's pair twice.  Per root r the count is
+#   star: C(K', 2) + sum_x C(n_x, 2) - P_r      pair: P_r - sum_x C(n_x, 2)
+# (window A_r of K slots, x in the root's nodes, n_x = slots of A_r whose
+# other end is x, K' = K - sum n_x, P_r = pairs of slots of A_r with the same
+# other end).  The roots of one anchor are the anchor's root list in
+# position order; their windows' ends only move forward, so
+#   sum over valid roots of P_r = sum over slots c of L of
+#       W(i_add) * add(c) - W(i_rem) * rem(c)
+# where c enters the window at root i_add (with add(c) = same-pair slots
+# already in it) and leaves at root i_rem (rem(c) = same-pair slots after it
+# still in it), and W(i) = valid roots at or after i.  Every term is O(log).
+WS_STAR, WS_PAIR = 1, 2
+
+
+def window_sweep_plan(md):
+    """(pattern, anchor motif node, anchor list is out) or None."""
+    if md["M"] != 3:
+        return None
+    k1, k2 = md["kind"][1], md["kind"][2]
+    a1, b1, a2, b2 = md["a"][1], md["b"][1], md["a"][2], md["b"][2]
+    if k1 == K_NEW_DST and k2 == K_NEW_DST and a1 == a2:
+        return (WS_STAR, a1, True)
+    if k1 == K_NEW_SRC and k2 == K_NEW_SRC and b1 == b2:
+        return (WS_STAR, b1, False)
+    if k1 == K_NEW_DST and k2 == K_CLOSE and (a2, b2) == (a1, b1):
+        return (WS_PAIR, a1, True)
+    if k1 == K_NEW_SRC and k2 == K_CLOSE and (a2, b2) == (a1, b1):
+        return (WS_PAIR, b1, False)
+    return None
+
+
+def _c2(n):
+    return n * (n - 1) // 2
+
+
+def window_sweep_count(g, md, delta, beg, end):
+    """The window-sweep kernel's count (mod 2^64), item by item: items are
+    edge positions q in [beg, max(end, qmax)); position q is a root (q < end)
+    and a slot of the anchor list of its source (out) or destination (in)."""
+    pat, anc, out = window_sweep_plan(md)
+    E, Eg, T = g["E"], g["Eg"], g["T"]
+    LR, LP = (g["outR"], g["outPos"]) if out else (g["inR"], g["inPos"])
+    RR, RP = (g["outR"], g["outPos"]) if anc == 0 else (g["inR"], g["inPos"])
+    same = (anc == 0) == out  # the anchor's root list is its window list
+
+    def pcnt(x, y, p, pend):  # slots of x's list whose other end is y, positions in (p, pend)
+        return edge_count(g, x, y, p, pend) if out else edge_count(g, y, x, p, pend)
+
+    if end <= beg:
+        return 0
+    qmax = pend_of(g, T[end - 1] + delta)
+    tot = 0
+    for q in range(beg, max(end, qmax)):
+        u, v = int(Eg[q, 0]), int(Eg[q, 1])
+        if q < end and u != v:  # root q
+            x = u if anc == 0 else v
+            pend = pend_of(g, T[q] + delta)
+            if same:
+                lo = (g["outSelf"] if out else g["inSelf"])[q] + 1
+            else:
+                lo = upper_bound(LP, LR[x], LR[x + 1], q)
+            hi = lower_bound(LP, lo, LR[x + 1], pend)
+            nu, nv = pcnt(x, u, q, pend), pcnt(x, v, q, pend)
+            c2 = _c2(nu) + _c2(nv)
+            tot += _c2(hi - lo - nu - nv) + c2 if pat == WS_STAR else -c2
+        if q > beg:  # slot of q in the list of x
+            x, w = (u, v) if out else (v, u)
+            r0 = lower_bound(RP, RR[x], RR[x + 1], beg)
+            r1 = lower_bound(RP, r0, RR[x + 1], end)
+            if r0 == r1:
+                continue
+            # first root whose window end reaches q: time >= T[q] - delta
+            tq = T[q] - delta
+            lo_, hi_ = r0, r1
+            while lo_ < hi_:
+                mid = (lo_ + hi_) >> 1
+                if T[RP[mid]] < tq:
+                    lo_ = mid + 1
+                else:
+                    hi_ = mid
+            ia = lo_
+            if ia == r1:
+                continue
+            P = RP[ia - 1] if ia > r0 else RP[r0]
+            if q <= P:
+                continue  # before the first root's window: in no window
+            add = pcnt(x, w, P, q)
+            if same:
+                c = (g["outSelf"] if out else g["inSelf"])[q]
+                ir = c if q < end else r1
+            else:
+                ir = lower_bound(RP, r0, r1, q)
+            rem = 0
+            if ir < r1:
+                rem = pcnt(x, w, q, pend_of(g, T[RP[ir]] + delta))
+
+            def valid(i):  # roots at slots [i, r1) that are not self-loops
+                n = r1 - i
+                if n > 0:
+                    n -= edge_count(g, x, x, RP[i] - 1, end)
+                return n
+
+            s = valid(ia) * add - (valid(ir) * rem if ir < r1 else 0)
+            tot += s if pat == WS_PAIR else -s
+    return tot % (1 << 64)

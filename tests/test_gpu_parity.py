@@ -418,3 +418,79 @@ def test_repeated_pair_rewrites():
         c, _ = dg.count_bytes(mot, delta)
         assert c == want
         dg.close()
+
+
+SWEEP_SHAPES = [[(0, 1), (0, 2), (0, 3)], [(0, 1), (2, 1), (3, 1)], [(0, 1), (1, 2), (1, 3)],
+                [(0, 1), (2, 0), (3, 0)], [(0, 1), (0, 2), (0, 2)], [(0, 1), (2, 1), (2, 1)],
+                [(0, 1), (1, 2), (1, 2)], [(0, 1), (2, 0), (2, 0)]]
+
+
+def test_window_sweep():
+    """3-edge stars and a new node's pair twice on one list of a root node go
+    to the window-sweep kernel: on graphs with self-loops, parallel edges and
+    tied times, over full and partial root ranges (empty ones too) and a
+    delta of 0, its counts equal the oracle and the mining kernel's
+    (sweep=-1); a batch mixing swept and mined queries, and the server, give
+    the same counts; an enumeration at a cap (whose cut counts root ranges
+    through the sweep) is the deterministic cut."""
+    abi = _abi()
+    rng = np.random.default_rng(23)
+    for case in range(48):
+        E = int(rng.integers(3, 2500))
+        N = int(rng.integers(2, 40))
+        U, V, T = random_graph(rng, E, N, int(rng.integers(1, max(2, E // 2))))
+        g = prep.preprocess(U, V, T)
+        edges = SWEEP_SHAPES[case % len(SWEEP_SHAPES)]
+        mot = [(a, b, i) for i, (a, b) in enumerate(edges)]
+        delta = int(rng.integers(0, 100)) if case % 6 else 0
+        dg = abi.DeviceGraph.from_dict(g)
+        nroots = g["E"] - 2
+        ranges = [(0, -1), (int(rng.integers(0, nroots)), int(rng.integers(0, nroots + 1))), (nroots // 2, nroots // 2)]
+        say("window sweep case", case, "E", g["E"], edges, "delta", delta, "ranges", ranges)
+        for beg, end in ranges:
+            want = oracle.count(g, mot, delta, beg, end if end >= 0 else nroots, nthreads=4) \
+                if (end < 0 or end > beg) else 0
+            got = dg.count(mot, delta, beg, end, tune=abi.tuning(sweep=1))
+            assert dg.last_stats()["sweep"] == 1
+            assert got == want, (case, edges, delta, beg, end, got, want)
+            assert dg.count(mot, delta, beg, end, tune=abi.tuning(sweep=-1)) == want
+            assert dg.last_stats()["sweep"] == 0
+            assert dg.count(mot, delta, beg, end) == want
+        want = oracle.count(g, mot, delta, nthreads=4)
+        other = [(0, 1), (1, 2), (2, 0)]
+        omot = [(a, b, i) for i, (a, b) in enumerate(other)]
+        got = dg.batch_count([(mot, delta), (omot, delta), (mot, delta + 7)], tune=abi.tuning(sweep=1))
+        assert got[0] == want and got[1] == oracle.count(g, omot, delta, nthreads=4)
+        assert got[2] == oracle.count(g, mot, delta + 7, nthreads=4)
+        if case % 8 == 0:
+            total, rows = oracle.enum(g, mot, delta, 1 << 20)
+            if total >= 3:
+                cap = max(1, total // 3)
+                cut, cut_cnt = deterministic_cut(g, rows, cap)
+                cnt, grows = dg.enum(mot, delta, cap=cap, tune=abi.tuning(sweep=1))
+                assert sorted(map(tuple, grows.tolist())) == sorted(cut) and cnt == cut_cnt, case
+        dg.close()
+
+
+def test_window_sweep_server():
+    """Swept queries through the server, alone and mixed with mined ones."""
+    abi = _abi()
+    rng = np.random.default_rng(29)
+    U, V, T = random_graph(rng, 3000, 60, 1500)
+    g = prep.preprocess(U, V, T)
+    dg = abi.DeviceGraph.from_dict(g)
+    qs = [([(a, b, i) for i, (a, b) in enumerate(e)], int(rng.integers(0, 200))) for e in SWEEP_SHAPES]
+    qs += [([(0, 1, 0), (1, 2, 1), (2, 0, 2)], 150), ([(0, 1, 0), (0, 2, 1), (2, 3, 2), (3, 0, 3)], 150)]
+    want = [oracle.count(g, m, d, nthreads=4) for m, d in qs]
+    say("window sweep server: %d queries" % len(qs))
+    srv = abi.Server(dg, max_batch=16, tune=abi.tuning(sweep=1))
+    tickets = [srv.submit(m, d, -1) for m, d in qs]
+    got = []
+    for t in tickets:
+        r = srv.wait(t)
+        assert r["status"] == 0
+        got.append(r["count"])
+        srv.release(t)
+    srv.close()
+    dg.close()
+    assert got == want
