@@ -1,15 +1,18 @@
 """Autotuner: launch parameters per motif shape class, chosen by measurement.
 
 A shape class (tm_shape_class: LDS stack depth and node-map width of a
-launch) fixes a launch's occupancy, so the best register cap, donation
-trigger and last-edge scan threshold are chosen per class: every candidate
-setting is timed on the class's heaviest queries among those that finish
-within a short screening limit (single-query launches, HIP-event kernel
-time, each capped by a time limit that counts as its time when hit), and the
-candidate with the smallest total wins (ties within 2% go to the default).  The
-table is saved as JSON and installed on a graph with tm_set_class_tuning,
-after which tm_count, tm_batch_count and the server use it whenever a call
-brings no launch parameters of its own.
+launch) fixes a launch's occupancy, so launch parameters are chosen per
+class.  Every query of the set is first timed once with the library
+defaults (single-query launches, HIP-event kernel time); each class's
+sample is its heaviest queries, the ones that decide a batch step's time.
+Then, per class, a coordinate descent over the axes below: starting from the
+defaults, each axis's values are tried in turn on top of the best setting so
+far, and a value is kept when it cuts the sample's total by more than 2%.  A
+candidate stops as soon as its running total (heaviest query first) can no
+longer win, and every run's count must equal the default run's.  The table
+is saved as JSON and installed on a graph with tm_set_class_tuning, after
+which tm_count, tm_batch_count and the server use it whenever a call brings
+no launch parameters of its own.
 
     python -m everest_amd.autotune --graph pl64m --out everest_amd/tuned/pl64m.json
 """
@@ -21,66 +24,87 @@ import time
 
 from . import abi
 
-# register cap (5: 96 VGPRs, 6: 80 VGPRs), donation trigger, scan threshold,
-# survivor compaction (the library default uses it where its LDS buffer costs
-# no resident wavefronts; the last three force it off or on)
-CANDIDATES = [
-    dict(waves_per_simd_hint=5),
-    dict(waves_per_simd_hint=6),
-    dict(waves_per_simd_hint=5, steal_after_us=250),
-    dict(waves_per_simd_hint=5, scan_max=32),
-    dict(waves_per_simd_hint=5, sb_flush=-1),
-    dict(waves_per_simd_hint=5, sb_flush=32, sb_inline=32),
-    dict(waves_per_simd_hint=5, sb_flush=48, sb_inline=48),
+# (field, values): the first value, 0, is the library default
+AXES = [
+    ("waves_per_simd_hint", [0, 6, 1]),     # register cap: 0 = 5 (96 VGPRs), 6 = 80 VGPRs, 1 = none
+    ("tail_swap", [0, 2, 4, 8, -1]),        # two-edge tail threshold (0 = 32, -1 = off)
+    ("sb_flush", [0, -1, 48, 64]),          # survivor compaction (0 = 32 where it costs no occupancy)
+    ("sb_inline", [0, 16, 48]),             # survivors a step expands itself with none buffered (0 = 32)
+    ("scan_max", [0, 8, 32]),               # last-edge ranges scanned up to this size (0 = 16)
+    ("claim_blocks", [0, 1, 2, 8, 32]),     # 256-root blocks per claim (0 = ~8 claims per wavefront)
+    ("waves_per_cu", [0, 16, 24]),          # resident wavefronts per CU (0 = occupancy-limited, <= 32)
+    ("steal_after_us", [0, 250, 4000]),     # donation trigger (0 = 1000 us)
+    ("steal_min", [0, 16, 256]),            # smallest donated half-frame (0 = 64 candidates)
+    ("steal_max", [0, 2, 8]),               # frames donated per look (0 = 4)
 ]
 
 
-def autotune(dg, queries, candidates=CANDIDATES, per_class=4, limit_ms=1500, screen_ms=300, log=None):
-    """queries: list of (edges as (a, b, k) triples, delta).  Returns
-    (table {class: tuning dict}, measurements {class: [(candidate, total_ms)]}).
+def _run(dg, m, d, tune):
+    """Kernel ms of one count, or None when it hit its time limit; and the count."""
+    try:
+        c = dg.count(m, d, tune=abi.tuning(**tune))
+        return dg.last_kernel_ms(), c
+    except abi.TMError as e:
+        if e.status != abi.TM_ERR_TIMEOUT:
+            raise
+        return None, None
 
-    The sample of a class is its per_class heaviest queries among those that
-    finish within screen_ms under the first candidate (the heavy queries
-    decide a batch's time; ones that do not finish would only measure the
-    limit)."""
+
+def autotune(dg, queries, axes=AXES, per_class=6, limit_ms=20000, log=None):
+    """queries: list of (edges as (a, b, k) triples, delta).  Returns
+    (table {class: tuning dict}, measurements {class: [(candidate, total_ms
+    or None when abandoned)]}, screen [(class, ms)] of the default pass)."""
+    base = {}
+    screen = []
+    for i, (m, d) in enumerate(queries):
+        ms, c = _run(dg, m, d, dict(time_limit_ms=limit_ms))
+        if ms is None:
+            raise RuntimeError("query %d exceeds %d ms with the defaults" % (i, limit_ms))
+        base[i] = (ms, c)
+        screen.append((abi.shape_class(m), round(ms, 3)))
+    if log:
+        log("default pass: %d queries, %.1f s of kernel time" % (len(queries), sum(v[0] for v in base.values()) / 1e3))
     by_class = {}
-    for m, d in queries:
-        by_class.setdefault(abi.shape_class(m), []).append((m, d))
+    for i, (m, d) in enumerate(queries):
+        by_class.setdefault(abi.shape_class(m), []).append(i)
     table, meas = {}, {}
     for cls in sorted(by_class):
-        qs = by_class[cls]
-        timed = []
-        for m, d in qs:
-            try:
-                dg.count(m, d, tune=abi.tuning(time_limit_ms=screen_ms, **candidates[0]))
-                timed.append((dg.last_kernel_ms(), m, d))
-            except abi.TMError as e:
-                if e.status != abi.TM_ERR_TIMEOUT:
-                    raise
-        timed.sort(key=lambda x: -x[0])
-        pick = [(m, d) for _, m, d in timed[:per_class]]
-        if not pick:
-            continue
-        rows = []
-        for cand in candidates:
-            total = 0.0
-            for m, d in pick:
-                try:
-                    dg.count(m, d, tune=abi.tuning(time_limit_ms=limit_ms, **cand))
-                    total += dg.last_kernel_ms()
-                except abi.TMError as e:
-                    if e.status != abi.TM_ERR_TIMEOUT:
-                        raise
-                    total += limit_ms
-            rows.append((cand, round(total, 3)))
-            if log:
-                log("class %d (%d queries, %d sampled): %s -> %.1f ms" % (cls, len(qs), len(pick), cand, total))
-        # ties within 2% go to the earlier candidate (the first is the library default)
-        best_ms = min(r[1] for r in rows)
-        best = next(c for c, t in rows if t <= best_ms * 1.02)
-        table[cls] = dict(best)
+        pick = sorted(by_class[cls], key=lambda i: -base[i][0])[:per_class]
+        best, best_ms = {}, sum(base[i][0] for i in pick)
+        rows = [({}, round(best_ms, 3))]
+        if log:
+            log("class %d: %d queries, sample %s, defaults %.1f ms" % (cls, len(by_class[cls]), pick, best_ms))
+        for field, values in axes:
+            for v in values:
+                if v == best.get(field, 0):
+                    continue
+                cand = dict(best)
+                if v == 0:
+                    cand.pop(field, None)
+                else:
+                    cand[field] = v
+                total = 0.0
+                for i in pick:  # heaviest first
+                    m, d = queries[i]
+                    lim = int(min(limit_ms, max(50.0, 3.0 * base[i][0])))
+                    ms, c = _run(dg, m, d, dict(cand, time_limit_ms=lim))
+                    if ms is not None and c != base[i][1]:
+                        raise RuntimeError("count of query %d differs under %s" % (i, cand))
+                    total += lim if ms is None else ms
+                    if total > best_ms * 1.02:
+                        break
+                won = total < best_ms * 0.98
+                rows.append((cand, round(total, 3) if total <= best_ms * 1.02 or won else None))
+                if log:
+                    log("class %d: %s -> %s" % (cls, cand, "%.1f ms" % total if rows[-1][1] is not None
+                                                  else "abandoned above %.1f ms" % (best_ms * 1.02)))
+                if won:
+                    best, best_ms = cand, total
+        table[cls] = best
         meas[cls] = rows
-    return table, meas
+        if log:
+            log("class %d: chose %s (%.1f ms)" % (cls, best, best_ms))
+    return table, meas, screen
 
 
 def apply(dg, table):
@@ -107,8 +131,7 @@ def main():
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--per-class", type=int, default=6)
-    ap.add_argument("--limit-ms", type=int, default=1500)
-    ap.add_argument("--screen-ms", type=int, default=300)
+    ap.add_argument("--limit-ms", type=int, default=20000)
     ap.add_argument("--out", default="")
     args = ap.parse_args()
     from . import core
@@ -124,13 +147,13 @@ def main():
                          hg.array("outEdgesR"), hg.array("Euid"))
     qs = [(queries.as_motif(e), d) for e, d in queries.batch_queries(args.queries, seed=args.seed)]
     t = time.time()
-    table, meas = autotune(dg, qs, per_class=args.per_class, limit_ms=args.limit_ms, screen_ms=args.screen_ms,
-                           log=log)
+    table, meas, screen = autotune(dg, qs, per_class=args.per_class, limit_ms=args.limit_ms, log=log)
     meta = dict(graph=args.graph, queries=args.queries, seed=args.seed, per_class=args.per_class,
-                limit_ms=args.limit_ms, screen_ms=args.screen_ms, seconds=round(time.time() - t, 1),
+                limit_ms=args.limit_ms, seconds=round(time.time() - t, 1), axes=[f for f, _ in AXES],
+                default_pass_ms=screen,
                 command="python -m everest_amd.autotune --graph %s --queries %d --seed %d --per-class %d "
-                        "--limit-ms %d --screen-ms %d --out <file>" % (
-                            args.graph, args.queries, args.seed, args.per_class, args.limit_ms, args.screen_ms))
+                        "--limit-ms %d --out <file>" % (args.graph, args.queries, args.seed, args.per_class,
+                                                        args.limit_ms))
     log("table", table)
     if args.out:
         save(args.out, table, meas, meta)
