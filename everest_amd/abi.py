@@ -38,7 +38,7 @@ class Tuning(ctypes.Structure):
                 ("steal_every", ctypes.c_int32), ("steal_max", ctypes.c_int32),
                 ("steal_min", ctypes.c_int32), ("steal_after_us", ctypes.c_int32),
                 ("waves_per_simd_hint", ctypes.c_int32), ("sb_flush", ctypes.c_int32),
-                ("sb_inline", ctypes.c_int32), ("tail_swap", ctypes.c_int32), ("fuse_classes", ctypes.c_int32),
+                ("sb_inline", ctypes.c_int32), ("tail_swap", ctypes.c_int32), ("batch_mode", ctypes.c_int32),
                 ("sweep", ctypes.c_int32)]
 
 

@@ -199,6 +199,16 @@ __device__ __forceinline__ void st_dev(unsigned long long *p, unsigned long long
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ int ld_dev(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the error bits of the launch just finished, kept for the caller (a batch
+// of single-query launches checks every launch's bits at the end)
+__global__ void error_kernel(const tm_sched *s, int *out) {
+  if (threadIdx.x == 0) *out = ld_dev(&s->error);
+}
+
 __global__ void reset_kernel(tm_sched *s, unsigned long long *counts, int *stops, int n, int grid) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < TM_STRIPES) st_dev(&s->next_chunk[i][0], 0ull);
@@ -1299,10 +1309,39 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
     key[q] = shape_class(num_edges[q], n);
     order[q] = q;
   }
+  const int mode = tune ? tune->batch_mode : 0;
+  std::vector<uint64_t> out(nq);
+  Slot &sl = g->slot[0];
+  if (mode <= 0) {
+    // one single-query launch per query, in call order, back to back on the
+    // slot's stream; one synchronization for the call.  The single-query
+    // kernel runs 6-11% faster per query than the multi-query one (a
+    // query index known at compile time), and on a work-dominated set the
+    // launches of light queries are a few percent of the time anyway.
+    tm_status st = reserve(g, sl, nq);
+    for (int q = 0; q < nq && st == TM_OK; q++) {
+      st = stage_queries(g, sl, q, 1, &q, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES, deltas, nullptr, nullptr);
+      if (st == TM_OK) st = launch(g, sl, q, 1, false, class_tuning(g, key[q], tune), 0, false);
+      if (st == TM_OK) {
+        hipLaunchKernelGGL(error_kernel, dim3(1), dim3(64), 0, sl.stream, sl.s.d_sched, sl.s.d_stops + q);
+        HIPTRY(hipGetLastError());
+      }
+    }
+    std::vector<int> err(nq);
+    if (st == TM_OK) st = finish(g, sl, 0, nq, counts);
+    if (st == TM_OK)
+      HIPTRY(hipMemcpy(err.data(), sl.s.d_stops, sizeof(int) * nq, hipMemcpyDeviceToHost));
+    if (st != TM_OK) return st;
+    int bits = 0;
+    for (int q = 0; q < nq; q++) bits |= err[q];
+    if (bits & ~4) return fail(TM_ERR_INTERNAL, "kernel invariant violated (error bits " + std::to_string(bits) + ")");
+    if (bits & 4) return fail(TM_ERR_TIMEOUT, "a query exceeded its time limit; counts discarded");
+    return TM_OK;
+  }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
   // fused: every query in one launch (its stack sized for the deepest class,
   // whose tuning entry it takes), at most TM_LAUNCH_MAXQ per launch
-  const bool fuse = tune && tune->fuse_classes > 0;
+  const bool fuse = mode >= 2;
   struct Group { int first, n; };
   std::vector<Group> groups;
   for (int i = 0; i < nq;) {
@@ -1311,8 +1350,6 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
     groups.push_back({i, j - i});
     i = j;
   }
-  std::vector<uint64_t> out(nq);
-  Slot &sl = g->slot[0];
   for (const Group &gr : groups) {
     tm_status st = stage_queries(g, sl, 0, gr.n, order.data() + gr.first, num_edges, msrc, mdst, TM_MAX_MOTIF_EDGES,
                                  deltas, nullptr, nullptr);

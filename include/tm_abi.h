@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define TM_ABI_VERSION 5
+#define TM_ABI_VERSION 6
 #define TM_MAX_MOTIF_EDGES 8
 #define TM_MAX_MOTIF_NODES 8
 
@@ -83,8 +83,10 @@ typedef struct tm_tuning {
                             second-to-last one brings, a partial match whose second-to-last edge has at
                             least this many candidates in the window is counted over the last edge's
                             candidates instead, if those are fewer; 0 = library default (32), < 0 = never */
-  int32_t fuse_classes;  /* tm_batch_count / tm_server: 1 = every query of the call in one persistent launch
-                            (its stack sized for the deepest motif), 0 = one launch per shape class */
+  int32_t batch_mode;    /* tm_batch_count: 0 = one single-query launch per query, back to back on one
+                            stream, one synchronization; 1 = one multi-query launch per shape class;
+                            2 = every query of the call in one multi-query launch (its stack sized for
+                            the deepest motif) */
   int32_t sweep;         /* counting, 3-edge motifs whose last two edges take candidates from one list of a
                             root node (a star of two new nodes, or a new node's pair twice): > 0 = counted by
                             the window-sweep kernel (O(E log E) per query whatever delta), < 0 = by the

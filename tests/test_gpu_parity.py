@@ -89,10 +89,13 @@ def test_batch_equals_single(small_cases):
     qs = [(random_motif(rng, int(rng.integers(2, 7))), int(rng.integers(0, 100))) for _ in range(40)]
     say("batch of", len(qs), "on case", i)
     got = dg.batch_count(qs)
+    per_class = dg.batch_count(qs, tune=abi.tuning(batch_mode=1))
+    fused = dg.batch_count(qs, tune=abi.tuning(batch_mode=2))
     single = [dg.count(m, d) for (m, d) in qs]
     want = [oracle.count(g, m, d, nthreads=4) for (m, d) in qs]
     dg.close()
     assert got == single == want
+    assert per_class == want and fused == want
 
 
 def test_zipf100k_all36():

@@ -82,7 +82,8 @@ def test_config3_deeper_motifs(pl64m, name, delta):
 
 def test_config4_all_256_batch_equals_single(pl64m):
     """All 256 queries of the batch workload: one tm_batch_count call equals
-    the 256 single-query counts; the two-edge tail off gives the same counts
+    the 256 single-query counts (single-query launches back to back, and one
+    multi-query launch per shape class); the two-edge tail off gives the same counts
     for the queries it applies to; oracle windows on a sample."""
     from everest_amd import abi
     from everest_amd.gen import queries
@@ -96,6 +97,8 @@ def test_config4_all_256_batch_equals_single(pl64m):
         single.append(dg.count(m, d))
     say("config4 batch of 256")
     assert dg.batch_count(motifs) == single
+    say("config4 batch of 256, one multi-query launch per shape class")
+    assert dg.batch_count(motifs, tune=abi.tuning(batch_mode=1)) == single
     rng = np.random.default_rng(44)
     for i in [int(x) for x in rng.choice(256, 8, replace=False)]:
         m, d = motifs[i]
