@@ -10,9 +10,10 @@ pass.
   batch   the whole set in one tm_batch_count call per step; value = queries/s
           of the batched step (every count checked bit-exact against plain)
   serve   Poisson arrivals at several rates (80% count, 20% enumerate with a
-          1M-row cap) through the server (continuous batching over launch
-          slots) and through a plain FIFO loop (one query per launch, in
-          arrival order), both run for real on the same stream
+          1M-row cap) through the server (continuous dispatch over two
+          launch slots, one request per launch, oldest first) and through a
+          plain FIFO loop (one query per launch, in arrival order, one at a
+          time), both run for real on the same stream
 
 roofline: the single-query counting kernel instantiation with the largest
 total time in the plain pass; achieved = the bytes its own search reads per
@@ -309,10 +310,12 @@ def serve(dg, motifs, counts, rate, args, n):
             c = dg.count(m, d)
             assert c == counts[pick[k]]
         plain_lat[k] = (time.perf_counter() - t0 - arrive[k]) * 1e3
+        if (k + 1) % 500 == 0:
+            log("  plain loop: %d/%d requests, %.1fs" % (k + 1, n, time.perf_counter() - t0))
     plain_span = time.perf_counter() - t0
 
-    # served: continuous batching over the server's launch slots
-    srv = abi.Server(dg, max_batch=256)
+    # served: continuous dispatch over the server's launch slots
+    srv = abi.Server(dg)
     tickets = [None] * n
     t0 = time.perf_counter()
 
@@ -345,6 +348,8 @@ def serve(dg, motifs, counts, rate, args, n):
         else:
             assert r["count"] == counts[pick[k]], "served count differs"
             srv.release(tickets[k])
+        if (k + 1) % 500 == 0:
+            log("  served: %d/%d requests, %.1fs" % (k + 1, n, time.perf_counter() - t0))
     th.join()
     st = srv.stats()
     srv.close()

@@ -331,9 +331,12 @@ tm_status tm_batch_free(tm_batch *b);
  *
  * Any host thread submits count or enumerate requests.  A dispatcher thread
  * forms launches from whatever is queued whenever one of the server's launch
- * slots is free (continuous batching): queued count requests of one
- * stack-depth class go together into one batch launch (at most max_batch),
- * oldest class first; an enumerate request runs alone and is admitted only
+ * slots is free (continuous dispatch), oldest request first: with max_batch
+ * 1 (the default) each count request is one single-query launch, so a light
+ * request overtakes a heavy one on another slot instead of waiting for it
+ * inside a batch; with max_batch > 1 the queued count requests of the oldest
+ * request's stack-depth class go together into one multi-query launch (at
+ * most max_batch); an enumerate request runs alone and is admitted only
  * once its row buffer (enum_cap * num_edges int32) fits in the enumeration
  * arena.  Launches on different slots run concurrently on the device, so
  * requests that arrive while others run overlap with them.  Each result is
@@ -345,7 +348,7 @@ typedef struct tm_server tm_server;
 
 typedef struct tm_server_config {
   int32_t slots;        /* launch slots used, 1 .. 4 (0: 2) */
-  int32_t max_batch;    /* count requests per launch, 1 .. 512 (0: 256) */
+  int32_t max_batch;    /* count requests per launch, 1 .. 512 (0: 1) */
   int64_t arena_bytes;  /* device memory for enumeration rows in flight (0: 1 GiB) */
   tm_tuning tune;       /* launch parameters of every launch */
 } tm_server_config;
