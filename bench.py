@@ -78,9 +78,15 @@ def main():
                          "pass, the metered pass and the K+W batch steps leave")
     ap.add_argument("--serve-only", action="store_true",
                     help="plain pass, then the serving sweep with --serve-requests per rate, no time budget")
+    ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no GPU: reduce two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of --plain-only "
+                         "--calibrate-copy into the per-launch HBM bytes of the roofline kernel (--json-out)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also copy this many bytes device-to-device (HBM counter scale for profiles)")
     args = ap.parse_args()
+    if args.reduce_pmc:
+        reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1], args.json_out)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
               "(--gpus %d requested; the multi-GPU path is design-only)" % args.gpus, file=sys.stderr)
@@ -132,7 +138,8 @@ def main():
         counts.append(c)
         st = dg.last_stats()
         nmap = 4 if max(max(a, b) for a, b, _ in m) + 1 <= 4 else 8
-        variant.append((nmap, st["wpe"]))
+        # queries counted by the window-sweep kernel are not mining-kernel launches
+        variant.append(("sweep",) if st["sweep"] else (nmap, st["wpe"]))
         if (i + 1) % 32 == 0:
             log("plain: %d/%d queries, %.1fs" % (i + 1, nq, time.perf_counter() - t))
     plain_total_s = sum(plain_ms) / 1e3
@@ -163,7 +170,7 @@ def main():
     groups = {}
     for i in range(nq):
         groups.setdefault(variant[i], []).append(i)
-    dom = max(groups, key=lambda k: sum(kernel_ms[i] for i in groups[k]))
+    dom = max((k for k in groups if k[0] != "sweep"), key=lambda k: sum(kernel_ms[i] for i in groups[k]))
     members = groups[dom]
     t = time.perf_counter()
     own = {}
@@ -180,15 +187,17 @@ def main():
     kname = "void tmk::mine_kernel<8, %d, false, 0, tmk::DynSpec, %d, false>(tm_kernel_args)" % dom
     log("dominant kernel %s: %d launches, %.4f GB and %.3f ms per launch -> %.1f GB/s" % (
         kname, dom_n, dom_bytes / 1e9, dom_ms, achieved))
+    # HBM bytes per launch from the committed counter runs of the same plain
+    # pass (bench.py --reduce-pmc), matched launch by launch in query order
     traffic, traffic_src = None, None
-    tf = os.path.join(ROOT, "profiles", "mine_kernel_pl64m_config4_hbm.json")
+    tf = os.path.join(ROOT, "profiles", "mine_kernel_pl64m_hbm.json")
     if os.path.exists(tf) and args.graph == "pl64m" and args.queries == 256 and args.seed == 4:
         with open(tf) as f:
             tj = json.load(f)
-        perq = tj.get("hbm_bytes_per_query", {})
-        if tj.get("kernel") == kname and all(str(i) in perq for i in members):
-            traffic = round(sum(perq[str(i)] for i in members) / dom_n, 1)
-            traffic_src = "profiles/mine_kernel_pl64m_config4_hbm.md (FETCH_SIZE x %.3f + WRITE_SIZE x %.3f)" % (
+        per = tj.get("hbm_bytes_per_launch", [])
+        if tj.get("kernel") == kname and len(per) == dom_n:
+            traffic = round(sum(per) / dom_n, 1)
+            traffic_src = "profiles/mine_kernel_pl64m_hbm.md (FETCH_SIZE x %.3f + WRITE_SIZE x %.3f)" % (
                 tj["fetch_scale"], tj["write_scale"])
 
     # ---- batched: the whole set in one call per step
@@ -392,6 +401,49 @@ def calibrate_copy(nbytes):
     b.copy_(a)
     torch.cuda.synchronize()
     log("calibration copy: %d bytes read, %d written" % (a.numel() * 4, b.numel() * 4))
+
+
+ROOFLINE_KERNEL = "void tmk::mine_kernel<8, 4, false, 0, tmk::DynSpec, 5, false>(tm_kernel_args)"
+
+
+def reduce_pmc(fetch_dir, write_dir, out):
+    """Per-launch HBM bytes of the roofline kernel from two counter runs of
+    the plain pass.  Each counter is scaled by what it reads for the 1 GiB
+    device-to-device calibration copy (the largest copyBuffer dispatch).
+    Launches with a one-wavefront grid (queries left to the window sweep:
+    the mining launch has no roots) are dropped; the rest are the plain
+    pass's launches of that kernel in query order."""
+    import csv
+    import glob
+
+    def rows(d, counter):
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if len(files) != 1:
+            raise SystemExit("expected one counter_collection.csv under %s, found %d" % (d, len(files)))
+        with open(files[0]) as f:
+            r = [x for x in csv.DictReader(f) if x["Counter_Name"] == counter]
+        r.sort(key=lambda x: int(x["Dispatch_Id"]))
+        cal = max(float(x["Counter_Value"]) for x in r if "copyBuffer" in x["Kernel_Name"])
+        ker = [float(x["Counter_Value"]) for x in r
+               if x["Kernel_Name"] == ROOFLINE_KERNEL and int(x["Grid_Size"]) > 64]
+        return cal, ker
+
+    fcal, fk = rows(fetch_dir, "FETCH_SIZE")
+    wcal, wk = rows(write_dir, "WRITE_SIZE")
+    if len(fk) != len(wk):
+        raise SystemExit("the two runs counted %d and %d launches" % (len(fk), len(wk)))
+    gib = float(1 << 30)
+    fscale, wscale = gib / (fcal * 1024.0), gib / (wcal * 1024.0)
+    per = [round(f * 1024.0 * fscale + w * 1024.0 * wscale, 1) for f, w in zip(fk, wk)]
+    res = {"kernel": ROOFLINE_KERNEL, "workload": "pl64m plain pass, 256 queries seed 4 (launches of this kernel)",
+           "fetch_scale": fscale, "write_scale": wscale, "launches": len(per), "hbm_bytes_per_launch": per,
+           "mean_hbm_bytes": round(sum(per) / max(1, len(per)), 1),
+           "fetch_bytes": round(sum(fk) * 1024.0 * fscale, 1), "write_bytes": round(sum(wk) * 1024.0 * wscale, 1)}
+    line = json.dumps(res)
+    if out:
+        with open(out, "w") as f:
+            f.write(line + "\n")
+    print(json.dumps({k: v for k, v in res.items() if k != "hbm_bytes_per_launch"}))
 
 
 def cpu_baseline(hg, arrs, motifs, dg, budget_s):
