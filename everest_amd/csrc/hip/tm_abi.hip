@@ -537,6 +537,7 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   tm_kernel_args a;
   memset(&a, 0, sizeof(a));
   a.g = g->view;
+  a.gv = g->d_view;
   a.queries = s.d_queries + q0;
   a.qtab = s.d_qtab + 4 * (size_t)q0;
   a.block_prefix = s.d_prefix + q0;
@@ -1081,6 +1082,11 @@ tm_status tm_graph_upload(int32_t device, const tm_host_graph *hg, tm_graph **ou
   TMPCHK(hipStreamSynchronize(s));
 #undef TMPCHK
   free_tmp();
+  // the view itself in device memory, for the kernels to read the graph through
+  if ((st = dev_alloc(g, &p, sizeof(tm_graph_view))) != TM_OK) return cleanup(st);
+  g->d_view = (tm_graph_view *)p;
+  if ((he = hipMemcpy(g->d_view, &g->view, sizeof(tm_graph_view), hipMemcpyHostToDevice)) != hipSuccess)
+    return cleanup(fail(TM_ERR_HIP, std::string("view upload: ") + hipGetErrorString(he)));
   *out = g;
   return TM_OK;
 }

@@ -67,6 +67,7 @@ struct tm_graph {
   std::vector<void *> allocs;
   int64_t bytes = 0;
   tm_graph_view view{};
+  tm_graph_view *d_view = nullptr;  // a device copy of view (the kernels read the graph through it)
   tmh::Slot slot[TM_SLOTS];  // slot 0 also carries the upload
   int last_slot = 0;         // slot of the last finished launch (stats, kernel time)
   tm_tuning class_tune[TM_NCLASSES] = {};  // per shape class (autotuner), used when a call brings none

@@ -765,7 +765,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   for (int q = lane; q < args.nq; q += WAVE) q_cnt[q] = 0;
   __syncthreads();
 
-  const tm_graph_view g = args.g;
+  const tm_graph_view &g = *args.gv;
   tm_sched *sched = args.sched;
   const int gen = ld_rlx(&sched->gen);  // advanced by the reset kernel before this launch
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));

@@ -196,4 +196,9 @@ typedef struct {
   int t2_min;                 // counting: frames of edge M-2 with at least this many candidates are summed
                               // over the last edge's candidates when that list is shorter (two-edge tail)
   int pad_t2;
+  // The same view as g, in device memory (written at upload).  The mining
+  // kernel reads the graph through it: a view held in the kernel's own
+  // variables ends up in per-lane scratch memory once the compiler selects
+  // between two of its fields' addresses (sunk loads of out- vs in-lists).
+  const tm_graph_view *gv;
 } tm_kernel_args;
