@@ -50,10 +50,13 @@ def _run(dg, m, d, tune):
         return None, None
 
 
-def autotune(dg, queries, axes=AXES, per_class=6, limit_ms=20000, log=None):
+def autotune(dg, queries, axes=AXES, per_class=6, limit_ms=20000, log=None, start=None, passes=2):
     """queries: list of (edges as (a, b, k) triples, delta).  Returns
     (table {class: tuning dict}, measurements {class: [(candidate, total_ms
-    or None when abandoned)]}, screen [(class, ms)] of the default pass)."""
+    or None when abandoned)]}, screen [(class, ms)] of the default pass).
+    start: an earlier table {class: tuning dict}; a class's entry there is
+    the descent's starting point when it beats the defaults.  The descent
+    repeats over the axes until a pass keeps no new value (at most passes)."""
     base = {}
     screen = []
     for i, (m, d) in enumerate(queries):
@@ -74,32 +77,55 @@ def autotune(dg, queries, axes=AXES, per_class=6, limit_ms=20000, log=None):
         rows = [({}, round(best_ms, 3))]
         if log:
             log("class %d: %d queries, sample %s, defaults %.1f ms" % (cls, len(by_class[cls]), pick, best_ms))
-        for field, values in axes:
-            for v in values:
-                if v == best.get(field, 0):
-                    continue
-                cand = dict(best)
-                if v == 0:
-                    cand.pop(field, None)
-                else:
-                    cand[field] = v
-                total = 0.0
-                for i in pick:  # heaviest first
-                    m, d = queries[i]
-                    lim = int(min(limit_ms, max(50.0, 3.0 * base[i][0])))
-                    ms, c = _run(dg, m, d, dict(cand, time_limit_ms=lim))
-                    if ms is not None and c != base[i][1]:
-                        raise RuntimeError("count of query %d differs under %s" % (i, cand))
-                    total += lim if ms is None else ms
-                    if total > best_ms * 1.02:
-                        break
-                won = total < best_ms * 0.98
-                rows.append((cand, round(total, 3) if total <= best_ms * 1.02 or won else None))
-                if log:
-                    log("class %d: %s -> %s" % (cls, cand, "%.1f ms" % total if rows[-1][1] is not None
-                                                  else "abandoned above %.1f ms" % (best_ms * 1.02)))
-                if won:
-                    best, best_ms = cand, total
+
+        def measure(cand, bound):
+            total = 0.0
+            for i in pick:  # heaviest first
+                m, d = queries[i]
+                lim = int(min(limit_ms, max(50.0, 3.0 * base[i][0])))
+                ms, c = _run(dg, m, d, dict(cand, time_limit_ms=lim))
+                if ms is not None and c != base[i][1]:
+                    raise RuntimeError("count of query %d differs under %s" % (i, cand))
+                total += lim if ms is None else ms
+                if total > bound:
+                    break
+            return total
+
+        first = dict((start or {}).get(cls, {}))
+        if first:
+            total = measure(first, best_ms * 1.02)
+            rows.append((first, round(total, 3) if total <= best_ms * 1.02 else None))
+            if log:
+                log("class %d: start %s -> %.1f ms" % (cls, first, total))
+            if total < best_ms:
+                best, best_ms = first, total
+        seen = {tuple(sorted(best.items()))}
+        for _ in range(passes):
+            moved = False
+            for field, values in axes:
+                for v in values:
+                    if v == best.get(field, 0):
+                        continue
+                    cand = dict(best)
+                    if v == 0:
+                        cand.pop(field, None)
+                    else:
+                        cand[field] = v
+                    key = tuple(sorted(cand.items()))
+                    if key in seen:
+                        continue
+                    seen.add(key)
+                    total = measure(cand, best_ms * 1.02)
+                    won = total < best_ms * 0.98
+                    rows.append((cand, round(total, 3) if total <= best_ms * 1.02 or won else None))
+                    if log:
+                        log("class %d: %s -> %s" % (cls, cand, "%.1f ms" % total if rows[-1][1] is not None
+                                                      else "abandoned above %.1f ms" % (best_ms * 1.02)))
+                    if won:
+                        best, best_ms = cand, total
+                        moved = True
+            if not moved:
+                break
         table[cls] = best
         meas[cls] = rows
         if log:
@@ -133,6 +159,7 @@ def main():
     ap.add_argument("--per-class", type=int, default=6)
     ap.add_argument("--limit-ms", type=int, default=20000)
     ap.add_argument("--out", default="")
+    ap.add_argument("--start", default="", help="an earlier table (JSON) whose entries start the descent")
     args = ap.parse_args()
     from . import core
     from .gen import graphs, queries
@@ -147,13 +174,14 @@ def main():
                          hg.array("outEdgesR"), hg.array("Euid"))
     qs = [(queries.as_motif(e), d) for e, d in queries.batch_queries(args.queries, seed=args.seed)]
     t = time.time()
-    table, meas, screen = autotune(dg, qs, per_class=args.per_class, limit_ms=args.limit_ms, log=log)
+    start = load(args.start)[0] if args.start else None
+    table, meas, screen = autotune(dg, qs, per_class=args.per_class, limit_ms=args.limit_ms, log=log, start=start)
     meta = dict(graph=args.graph, queries=args.queries, seed=args.seed, per_class=args.per_class,
                 limit_ms=args.limit_ms, seconds=round(time.time() - t, 1), axes=[f for f, _ in AXES],
                 default_pass_ms=screen,
                 command="python -m everest_amd.autotune --graph %s --queries %d --seed %d --per-class %d "
-                        "--limit-ms %d --out <file>" % (args.graph, args.queries, args.seed, args.per_class,
-                                                        args.limit_ms))
+                        "--limit-ms %d%s --out <file>" % (args.graph, args.queries, args.seed, args.per_class,
+                                                          args.limit_ms, " --start " + args.start if args.start else ""))
     log("table", table)
     if args.out:
         save(args.out, table, meas, meta)

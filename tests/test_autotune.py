@@ -54,3 +54,14 @@ def test_count_change_is_an_error():
     fg = FakeGraph({("tail_swap", 2): 0.5}, wrong=("tail_swap", 2))
     with pytest.raises(RuntimeError, match="differs"):
         autotune.autotune(fg, QS, per_class=2)
+
+
+def test_start_table_and_second_pass():
+    # a start entry better than the defaults is where the descent begins,
+    # and the descent still adds the values that pay on top of it
+    fg = FakeGraph({("tail_swap", 4): 0.5, ("claim_blocks", 2): 0.7})
+    start = {autotune.abi.shape_class(m): {"claim_blocks": 2} for m, _ in QS}
+    table, meas, _ = autotune.autotune(fg, QS, per_class=2, start=start)
+    for cls, t in table.items():
+        assert t == {"tail_swap": 4, "claim_blocks": 2}, (cls, t)
+        assert meas[cls][1][0] == {"claim_blocks": 2}
