@@ -26,7 +26,7 @@ from . import abi
 
 # (field, values): the first value, 0, is the library default
 AXES = [
-    ("waves_per_simd_hint", [0, 6, 1]),     # register cap: 0 = 5 (96 VGPRs), 6 = 80 VGPRs, 1 = none
+    ("waves_per_simd_hint", [0, 6, 4, 1]),  # register cap: 0 = 5 (96 VGPRs), 6 = 80, 4 = 128, 1 = none
     ("tail_swap", [0, 2, 4, 8, -1]),        # two-edge tail threshold (0 = 32, -1 = off)
     ("sb_flush", [0, -1, 48, 64]),          # survivor compaction (0 = 32 where it costs no occupancy)
     ("sb_inline", [0, 16, 48]),             # survivors a step expands itself with none buffered (0 = 32)

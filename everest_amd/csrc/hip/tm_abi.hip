@@ -459,9 +459,11 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
     const int roots = (counting ? 4 : 2) * tmk::TM_ROOTS;    // window, and the root-pair list (counting)
     return (frames * per_frame + roots + sb + t2) * (int)sizeof(int) + nq * (int)sizeof(unsigned long long);
   };
-  int wpe = 0;  // register cap of the counting kernel: 1 (none), 5 (96 VGPRs) or 6 (80 VGPRs)
-  if (tune && tune->waves_per_simd_hint > 0)
-    wpe = tune->waves_per_simd_hint >= 6 ? 6 : (tune->waves_per_simd_hint == 5 ? 5 : 1);
+  int wpe = 0;  // register cap of the counting kernel: 1 (none), 4 (128 VGPRs), 5 (96) or 6 (80)
+  if (tune && tune->waves_per_simd_hint > 0) {
+    const int h = tune->waves_per_simd_hint;
+    wpe = h >= 6 ? 6 : h == 5 ? 5 : h == 4 ? 4 : 1;
+  }
   if (!fn && wpe == 0) {
     wpe = 5;  // measured best on the 64M-edge set over 15 heavy queries (5: 7.00 s, 6: 7.81 s, 1: 7.68 s);
               // the autotuner picks 6 per shape where it wins

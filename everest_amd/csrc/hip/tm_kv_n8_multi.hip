@@ -1,5 +1,6 @@
 // Multi-query (batch) counting variants, node map of 8 (see tm_variants.h).
 #include "tm_variants.cuh"
 TMV_DEF(8, 0, 0, 1, 1)
+TMV_DEF(8, 0, 0, 4, 1)
 TMV_DEF(8, 0, 0, 5, 1)
 TMV_DEF(8, 0, 0, 6, 1)

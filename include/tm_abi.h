@@ -72,8 +72,9 @@ typedef struct tm_tuning {
   int32_t steal_min;     /* a donated half-frame holds at least this many candidates */
   int32_t steal_after_us; /* a wavefront donates only after holding one task this long */
   int32_t waves_per_simd_hint; /* counting kernel variant: 1 = compiler's register allocation,
-                                  5 = capped at 96 VGPRs (20 wavefronts per CU), 6 = capped at 80
-                                  VGPRs (24 per CU); 0 = 5 */
+                                  4 = capped at 128 VGPRs (16 wavefronts per CU, no spills),
+                                  5 = capped at 96 VGPRs (20 per CU), 6 = capped at 80 VGPRs
+                                  (24 per CU); 0 = 5 */
   int32_t sb_flush;      /* counting: candidates that pass the filters of one level are buffered in
                             LDS and expanded together once this many (1..64) are held; -1 = each
                             step expands its own survivors; 0 = the library's choice */
