@@ -403,39 +403,51 @@ def calibrate_copy(nbytes):
     log("calibration copy: %d bytes read, %d written" % (a.numel() * 4, b.numel() * 4))
 
 
-ROOFLINE_KERNEL = "void tmk::mine_kernel<8, 4, false, 0, tmk::DynSpec, 5, false>(tm_kernel_args)"
-
-
 def reduce_pmc(fetch_dir, write_dir, out):
     """Per-launch HBM bytes of the roofline kernel from two counter runs of
-    the plain pass.  Each counter is scaled by what it reads for the 1 GiB
+    the plain pass.  The roofline kernel is the single-query counting
+    variant (`mine_kernel<..., false, 0, ..., false>`) with the largest
+    total duration in the FETCH_SIZE run, as bench.py picks it by kernel
+    time.  Each counter is scaled by what it reads for the 1 GiB
     device-to-device calibration copy (the largest copyBuffer dispatch).
     Launches with a one-wavefront grid (queries left to the window sweep:
     the mining launch has no roots) are dropped; the rest are the plain
     pass's launches of that kernel in query order."""
     import csv
     import glob
+    import re
 
-    def rows(d, counter):
+    def load(d, counter):
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if len(files) != 1:
             raise SystemExit("expected one counter_collection.csv under %s, found %d" % (d, len(files)))
         with open(files[0]) as f:
             r = [x for x in csv.DictReader(f) if x["Counter_Name"] == counter]
         r.sort(key=lambda x: int(x["Dispatch_Id"]))
-        cal = max(float(x["Counter_Value"]) for x in r if "copyBuffer" in x["Kernel_Name"])
-        ker = [float(x["Counter_Value"]) for x in r
-               if x["Kernel_Name"] == ROOFLINE_KERNEL and int(x["Grid_Size"]) > 64]
-        return cal, ker
+        return r
 
-    fcal, fk = rows(fetch_dir, "FETCH_SIZE")
-    wcal, wk = rows(write_dir, "WRITE_SIZE")
+    fr, wr = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    counting = re.compile(r"^void tmk::mine_kernel<8, \d, false, 0, tmk::DynSpec, \d, false>\(tm_kernel_args\)$")
+    total = {}
+    for x in fr:
+        if counting.match(x["Kernel_Name"]) and int(x["Grid_Size"]) > 64:
+            total[x["Kernel_Name"]] = total.get(x["Kernel_Name"], 0) + int(x["End_Timestamp"]) - int(x["Start_Timestamp"])
+    if not total:
+        raise SystemExit("no single-query counting launches in %s" % fetch_dir)
+    kname = max(total, key=total.get)
+
+    def per_launch(r):
+        cal = max(float(x["Counter_Value"]) for x in r if "copyBuffer" in x["Kernel_Name"])
+        return cal, [float(x["Counter_Value"]) for x in r if x["Kernel_Name"] == kname and int(x["Grid_Size"]) > 64]
+
+    fcal, fk = per_launch(fr)
+    wcal, wk = per_launch(wr)
     if len(fk) != len(wk):
         raise SystemExit("the two runs counted %d and %d launches" % (len(fk), len(wk)))
     gib = float(1 << 30)
     fscale, wscale = gib / (fcal * 1024.0), gib / (wcal * 1024.0)
     per = [round(f * 1024.0 * fscale + w * 1024.0 * wscale, 1) for f, w in zip(fk, wk)]
-    res = {"kernel": ROOFLINE_KERNEL, "workload": "pl64m plain pass, 256 queries seed 4 (launches of this kernel)",
+    res = {"kernel": kname, "workload": "pl64m plain pass, 256 queries seed 4 (launches of this kernel)",
            "fetch_scale": fscale, "write_scale": wscale, "launches": len(per), "hbm_bytes_per_launch": per,
            "mean_hbm_bytes": round(sum(per) / max(1, len(per)), 1),
            "fetch_bytes": round(sum(fk) * 1024.0 * fscale, 1), "write_bytes": round(sum(wk) * 1024.0 * wscale, 1)}
