@@ -246,6 +246,24 @@ __device__ __forceinline__ int pair_count(const int *__restrict__ pnbr, const in
   return c1 - c0;
 }
 
+// The last position in (p, pend) of an edge x -> y, or -1 (the pair's
+// positions through the shorter of out(x) and in(y)).
+__device__ __forceinline__ int pair_last(const tm_graph_view &g, int x, int y, int p, int pend,
+                                         Meter *mt = nullptr) {
+  int lo = g.outR[x], hi = g.outR[x + 1], key = y;
+  const int *pn = g.outPairNbr, *pp = g.outPairPos;
+  const int ilo = g.inR[y], ihi = g.inR[y + 1];
+  if (mt) mt->b += 16;
+  if (ihi - ilo < hi - lo) { lo = ilo; hi = ihi; key = x; pn = g.inPairNbr; pp = g.inPairPos; }
+  const int s0 = lower_bound_i(pn, lo, hi, key, mt);
+  if (mt && s0 < hi) mt->b += 4;
+  if (s0 >= hi || pn[s0] != key) return -1;
+  const int s1 = gallop_ub(pn, s0 + 1, hi, key, mt);
+  const int i = lower_bound_i(pp, s0, s1, pend, mt) - 1;
+  if (mt && i >= s0) mt->b += 4;
+  return (i >= s0 && pp[i] > p) ? pp[i] : -1;
+}
+
 // Edges x -> y with positions in (p, pend), searched in the shorter of out(x)
 // and in(y).
 __device__ __forceinline__ int edge_count(const tm_graph_view &g, int x, int y, int p, int pend,
@@ -1026,7 +1044,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       m[0] = u; m[1] = v;
       CandList L{g.outPos, g.outNbr, 0, 0, 0};
       const long long tl = (long long)t + q.delta;
-      const int tli = tl > 0x7fffffffll ? 0x7fffffff : (int)tl;
+      int tli = tl > 0x7fffffffll ? 0x7fffffff : (int)tl;
       if (ok) {
         L = cand_start<SP>(g, md, 1, m, r, mt);
         if (mt && L.lo < L.hi) meter.b += 8;
@@ -1043,6 +1061,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       if (ok) {
         pend = pend_of(g, tl, mt);
         if (INSTR != 1) ok = (pend - r - 1) >= (M - 1);
+      }
+      // (counting) the last motif edge closes the root's own two nodes: every
+      // edge of a match lies at or before that pair's last edge in the
+      // window, so the root's window ends there (positions and pruning time)
+      if (T2 && ok && M >= 3 && SP::kind(md, M - 1) == TM_K_CLOSE && SP::a(md, M - 1) < 2 &&
+          SP::b(md, M - 1) < 2) {
+        const int x = SP::a(md, M - 1) == 0 ? u : v, y = SP::b(md, M - 1) == 0 ? u : v;
+        const int pl = pair_last(g, x, y, r, pend, mt);
+        if (pl < 0) {
+          ok = false;
+        } else {
+          pend = pl + 1;
+          if (mt) meter.b += 4;
+          tli = g.T[pl];
+          ok = (pend - r - 1) >= (M - 1);
+        }
       }
       const int last = ENUM ? M - 1 : M - 2;  // last motif edge handled per candidate
       if (last == 0) {

@@ -351,6 +351,21 @@ def pair_count(pn, pp, lo, hi, key, p, pend):
     return c1 - c0
 
 
+def pair_last(g, x, y, p, pend):
+    """The last position in (p, pend) of an edge x -> y, or -1."""
+    olo, ohi = g["outR"][x], g["outR"][x + 1]
+    ilo, ihi = g["inR"][y], g["inR"][y + 1]
+    lo, hi, key, pn, pp = olo, ohi, y, g["outPairNbr"], g["outPairPos"]
+    if ihi - ilo < ohi - olo:
+        lo, hi, key, pn, pp = ilo, ihi, x, g["inPairNbr"], g["inPairPos"]
+    s0 = lower_bound(pn, lo, hi, key)
+    if s0 >= hi or pn[s0] != key:
+        return -1
+    s1 = gallop_ub(pn, s0 + 1, hi, key)
+    i = lower_bound(pp, s0, s1, pend) - 1
+    return pp[i] if i >= s0 and pp[i] > p else -1
+
+
 def edge_count(g, x, y, p, pend):
     olo, ohi = g["outR"][x], g["outR"][x + 1]
     ilo, ihi = g["inR"][y], g["inR"][y + 1]
@@ -530,13 +545,25 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
                 pend = pend_of(g, t + delta)
                 if pend - r - 1 < M - 1 and not instr:
                     continue
+                tl = t + delta
+                if (not enumerate_ and not instr and M >= 3 and md["kind"][M - 1] == K_CLOSE
+                        and md["a"][M - 1] < 2 and md["b"][M - 1] < 2):
+                    # the last edge closes the root's nodes: the window ends at that pair's last edge
+                    x = u if md["a"][M - 1] == 0 else v
+                    y = u if md["b"][M - 1] == 0 else v
+                    pl = pair_last(g, x, y, r, pend)
+                    if pl < 0:
+                        continue
+                    pend, tl = pl + 1, g["T"][pl]
+                    if pend - r - 1 < M - 1:
+                        continue
                 if last == 0:
                     cnt += leaf_count(g, md, 1, m, L, r, pend, scan_max)
                     continue
                 cb, d = L[2], L[4]
                 ce = gallop_lb(L[0], L[2], L[3], pend)
                 if ce > cb:
-                    pushes.append(dict(cur=cb, end=ce, p=r, pend=pend, lvl=0, dir=d, m=m[:], e=[r], tl=t + delta))
+                    pushes.append(dict(cur=cb, end=ce, p=r, pend=pend, lvl=0, dir=d, m=m[:], e=[r], tl=tl))
             assert top + len(pushes) <= FR
             for f in pushes:
                 if not enumerate_ and not instr and M == 3:

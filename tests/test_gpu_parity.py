@@ -497,3 +497,30 @@ def test_window_sweep_server():
     srv.close()
     dg.close()
     assert got == want
+
+
+def test_window_ends_at_last_root_pair_edge():
+    """Motifs whose last edge closes the root's two nodes (the root's window
+    ends at that pair's last edge in it): counts equal the oracle with the
+    two-edge tail forced and off, through the metered kernel and in a batch."""
+    abi = _abi()
+    rng = np.random.default_rng(73)
+    shapes = [[(0, 1), (0, 2), (3, 2), (0, 1)], [(0, 1), (1, 2), (0, 1)], [(0, 1), (2, 0), (1, 0)],
+              [(0, 1), (0, 2), (2, 1), (1, 0)], [(0, 1), (2, 1), (2, 3), (0, 1)],
+              [(0, 1), (0, 2), (3, 2), (2, 3), (0, 1)]]
+    for case in range(36):
+        E = int(rng.integers(50, 2500))
+        U, V, T = random_graph(rng, E, int(rng.integers(2, 30)), int(rng.integers(E // 8 + 1, E)))
+        g = prep.preprocess(U, V, T)
+        edges = shapes[case % len(shapes)]
+        mot = [(a, b, i) for i, (a, b) in enumerate(edges)]
+        delta = int(rng.integers(0, 200))
+        say("last root-pair edge case", case, "E", g["E"], edges, "delta", delta)
+        want = oracle.count(g, mot, delta, nthreads=4)
+        dg = abi.DeviceGraph.from_dict(g)
+        for ts in (1, -1):
+            assert dg.count(mot, delta, tune=abi.tuning(tail_swap=ts)) == want, (case, ts)
+        c, _ = dg.count_bytes(mot, delta)
+        assert c == want
+        assert dg.batch_count([(mot, delta), (mot, delta)]) == [want, want]
+        dg.close()

@@ -168,3 +168,29 @@ def test_model_concurrent_persistent_launches():
     for trial in range(20):
         work = [[int(x) for x in rng.integers(0, 6, cap)]]
         assert km.run_concurrent_launches([cap], cap, 1 << 30, work, rng) == "done"
+
+
+def test_model_window_ends_at_last_root_pair_edge():
+    """A motif whose last edge closes the root's two nodes: the root's window
+    ends at that pair's last edge inside it (positions and pruning time);
+    the counts stay the oracle's, with and without the two-edge tail."""
+    rng = np.random.default_rng(71)
+    shapes = [[(0, 1), (0, 2), (3, 2), (0, 1)], [(0, 1), (1, 2), (0, 1)], [(0, 1), (2, 0), (1, 0)],
+              [(0, 1), (0, 2), (2, 1), (1, 0)], [(0, 1), (2, 1), (2, 3), (0, 1)]]
+    nz = 0
+    for case in range(30):
+        E = int(rng.integers(20, 220))
+        U, V, T = random_graph(rng, E, int(rng.integers(2, 10)), int(rng.integers(5, 120)))
+        g = prep.preprocess(U, V, T)
+        gv = km.build_view(g)
+        edges = shapes[case % len(shapes)]
+        md = km.build_motif(edges)
+        mot = [(a, b, i) for i, (a, b) in enumerate(edges)]
+        M = len(edges)
+        delta = int(rng.integers(0, 60))
+        want = oracle.count(g, mot, delta, nthreads=1)
+        nz += want > 0
+        for t2 in (None, 1):
+            got, _, _ = km.run_wave(gv, md, delta, 0, g["E"] - M + 1, t2_min=t2)
+            assert got == want, (case, edges, delta, t2, got, want)
+    assert nz > 10
