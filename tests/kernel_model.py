@@ -723,14 +723,41 @@ def run_wave(g, md, delta, beg, end, enumerate_=False, scan_max=16, chunk_roots=
 STRIPES = 8
 
 
-class _Shared:
-    def __init__(self, nchunks, pool_cap):
-        self.next_chunk = [0] * STRIPES   # striped chunk counters
-        self.nchunks = nchunks
+class Ring:
+    """The donation ring and its release words, kept across launches as the
+    device keeps them: records are tagged with the launch generation, so a
+    record or release left by an earlier launch never matches."""
+
+    def __init__(self, pool_cap, gen_bits=18):
         assert pool_cap & (pool_cap - 1) == 0
         self.cap = pool_cap
-        self.ring = [None] * pool_cap   # slot -> (record index, frame)
-        self.done = [-1] * pool_cap     # slot -> last lap taken
+        self.slots = [None] * pool_cap   # slot -> (generation, record index, frame)
+        self.done = [None] * pool_cap    # slot -> (generation, lap) of the last record taken
+        self.gen = 0                     # device generation (advanced by each launch's reset)
+        self.host_gen = 0                # launches since the ring was last cleared (host count)
+        self.gen_bits = gen_bits
+
+    def launch(self):
+        """Host side of a launch (count_launch) then the reset kernel's increment."""
+        if self.host_gen + 1 >= (1 << self.gen_bits):
+            # generations wrap: clear the ring so no stale tag can match
+            self.slots = [None] * self.cap
+            self.done = [None] * self.cap
+            self.gen = 0
+            self.host_gen = 0
+        self.host_gen += 1
+        self.gen = (self.gen + 1) % (1 << self.gen_bits)
+
+
+class _Shared:
+    def __init__(self, nchunks, pool_cap, ring=None):
+        self.next_chunk = [0] * STRIPES   # striped chunk counters
+        self.nchunks = nchunks
+        if ring is None:
+            ring = Ring(pool_cap)
+            ring.launch()
+        self.ringst = ring
+        self.cap = ring.cap
         self.head = 0           # tickets drawn
         self.tail = 0           # records reserved
         self.ndonated = 0
@@ -739,6 +766,16 @@ class _Shared:
         self.idle_cap = 1 << 30
         self.busy = 0
         self.idle = 0
+        self.abort = False      # a protocol wait gave up (error bit 8): every wavefront leaves
+        self.error = 0
+        self.spin_limit = None  # bounded wait of a donor on its ring slot (None: unbounded)
+
+    def record(self, slot):
+        return self.ringst.slots[slot]
+
+    def done_lap(self, slot):
+        d = self.ringst.done[slot]
+        return d[1] if d is not None and d[0] == self.ringst.gen else -1
 
 
 class Wave:
@@ -767,13 +804,23 @@ class Wave:
         self.home = Wave.count % STRIPES
         Wave.count += 1
         self.stripe_off = 0
+        self.spins = 0
         sh.busy += 1
 
     def step(self):
-        if self.pending is not None:  # spinning on a ring slot
-            self.write_record()
+        sh = self.sh
+        if sh.abort:  # a wait gave up somewhere: leave (the launch reports an error)
+            self.done = True
             return
-        g, md, sh = self.g, self.md, self.sh
+        if self.pending is not None:  # spinning on a ring slot
+            if not self.write_record():
+                self.spins += 1
+                if sh.spin_limit is not None and self.spins > sh.spin_limit:
+                    sh.error |= 8
+                    sh.abort = True
+                    self.done = True
+            return
+        g, md = self.g, self.md
         M = md["M"]
         last = M - 2
         if self.top == 0 and self.roots_done:
@@ -797,12 +844,13 @@ class Wave:
                 sh.head += 1
                 return
             slot, lap = self.ticket % sh.cap, self.ticket // sh.cap
-            if sh.ring[slot] is not None and sh.ring[slot][0] == self.ticket:
+            rec = sh.record(slot)
+            if rec is not None and rec[0] == sh.ringst.gen and rec[1] == self.ticket:
                 assert self.ticket not in sh.taken
                 sh.taken.add(self.ticket)
-                f = sh.ring[slot][1]
-                assert sh.done[slot] == lap - 1
-                sh.done[slot] = lap  # the slot's next lap may be written
+                f = rec[2]
+                assert sh.done_lap(slot) == lap - 1
+                sh.ringst.done[slot] = (sh.ringst.gen, lap)  # the slot's next lap may be written
                 self.ticket = None
                 sh.waiters -= 1
                 self.waiter = False
@@ -952,28 +1000,34 @@ class Wave:
         sh, frames = self.sh, self.frames
         r, b = self.pending
         slot, lap = r % sh.cap, r // sh.cap
-        if lap > 0 and sh.done[slot] != lap - 1:
+        if lap > 0 and sh.done_lap(slot) != lap - 1:
             return False
         f = dict(frames[b])
         mid = f["cur"] + (f["end"] - f["cur"]) // 2
         don = dict(f)
         don["cur"] = mid
-        sh.ring[slot] = (r, don)
+        sh.ringst.slots[slot] = (sh.ringst.gen, r, don)
         sh.ndonated += 1
         f["end"] = mid
         frames[b] = f
         self.pending = None
+        self.spins = 0
         return True
 
 
 def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, steal_every=1, steal_max=4,
-              pool_cap=1 << 20, seed=0, idle_cap=1 << 30):
+              pool_cap=1 << 20, seed=0, idle_cap=1 << 30, ring=None, spin_limit=None, with_error=False):
     """Simulate nwaves wavefronts stepping in a random interleaving; returns
-    (count, number of donated frames)."""
+    (count, number of donated frames) [, error bits].  ring: a Ring kept
+    from earlier launches (this launch is counted on it first);
+    spin_limit: the donor's bounded wait on its ring slot."""
     rng = np.random.default_rng(seed)
     nchunks = max(0, (end - beg + chunk_roots - 1) // chunk_roots)
-    sh = _Shared(nchunks, pool_cap)
+    if ring is not None:
+        ring.launch()
+    sh = _Shared(nchunks, pool_cap, ring)
     sh.idle_cap = idle_cap
+    sh.spin_limit = spin_limit
     waves = [Wave(g, md, delta, beg, end, sh, scan_max, chunk_roots, steal_every, steal_max)
              for _ in range(nwaves)]
     guard = 0
@@ -983,9 +1037,11 @@ def run_waves(g, md, delta, beg, end, nwaves=4, scan_max=16, chunk_roots=64, ste
         w.step()
         guard += 1
         assert guard < 10_000_000, "no progress"
-    assert len(sh.taken) == sh.tail == sh.ndonated, "a donated record was never taken"
-    assert sh.busy == 0 and sh.idle == 0 and sh.waiters == 0
-    return sum(w.cnt for w in waves), sh.ndonated
+    if not sh.abort:
+        assert len(sh.taken) == sh.tail == sh.ndonated, "a donated record was never taken"
+        assert sh.busy == 0 and sh.idle == 0 and sh.waiters == 0
+    res = (sum(w.cnt for w in waves), sh.ndonated)
+    return res + (sh.error,) if with_error else res
 
 
 # ------------------------------------------------ two-edge tail by the shorter list

@@ -194,3 +194,53 @@ def test_model_window_ends_at_last_root_pair_edge():
             got, _, _ = km.run_wave(gv, md, delta, 0, g["E"] - M + 1, t2_min=t2)
             assert got == want, (case, edges, delta, t2, got, want)
     assert nz > 10
+
+
+def test_model_donor_give_up_aborts_every_wavefront():
+    """A donor whose bounded wait on its ring slot gives up raises error bit 8
+    and the abort flag, and every wavefront then leaves (no wavefront keeps
+    waiting for the record that will never be written); runs that do not
+    give up count exactly."""
+    rng = np.random.default_rng(37)
+    gave_up = exact = 0
+    for _ in range(30):
+        U, V, T = random_graph(rng, int(rng.integers(80, 300)), int(rng.integers(2, 6)), int(rng.integers(1, 20)))
+        M = int(rng.integers(3, 6))
+        mot = random_motif(rng, M)
+        delta = int(rng.integers(5, 40))
+        g = prep.preprocess(U, V, T)
+        gv = km.build_view(g)
+        md = km.build_motif([(a, b) for a, b, _ in mot])
+        got, _, err = km.run_waves(gv, md, delta, 0, g["E"] - M + 1, nwaves=int(rng.integers(3, 7)),
+                                   steal_every=1, steal_max=4, pool_cap=1, seed=int(rng.integers(0, 1 << 30)),
+                                   spin_limit=int(rng.choice([0, 1, 3])), with_error=True)
+        if err:
+            assert err == 8
+            gave_up += 1
+        else:
+            assert got == oracle.count(g, mot, delta, nthreads=1)
+            exact += 1
+    assert gave_up > 0 and exact > 0
+
+
+def test_model_ring_across_consecutive_launches():
+    """Consecutive launches share the donation ring and its release words, as
+    on the device: records and releases are tagged with the launch
+    generation, the host clears the ring before the generation wraps (2-bit
+    generations here, so it wraps every 3 launches), and every launch counts
+    exactly with small rings that force many laps."""
+    rng = np.random.default_rng(41)
+    U, V, T = random_graph(rng, 300, 5, 25)
+    g = prep.preprocess(U, V, T)
+    gv = km.build_view(g)
+    for cap in (1, 2, 4):
+        ring = km.Ring(cap, gen_bits=2)
+        for launch in range(10):
+            M = int(rng.integers(3, 6))
+            mot = random_motif(rng, M)
+            md = km.build_motif([(a, b) for a, b, _ in mot])
+            delta = int(rng.integers(5, 40))
+            got, _ = km.run_waves(gv, md, delta, 0, g["E"] - M + 1, nwaves=int(rng.integers(2, 6)),
+                                  steal_every=1, steal_max=int(rng.integers(1, 5)),
+                                  seed=int(rng.integers(0, 1 << 30)), ring=ring)
+            assert got == oracle.count(g, mot, delta, nthreads=1), (cap, launch)
