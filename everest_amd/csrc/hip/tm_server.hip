@@ -4,9 +4,10 @@
 // Requests are queued by any host thread.  One dispatcher thread owns the
 // graph while the server lives: it polls the launch slots, and whenever a
 // slot is free it forms the next launch from what is queued (continuous
-// dispatch), oldest request first.  A count request is one single-query
-// launch by default; with max_batch > 1, the waiting count requests of the
-// oldest one's stack-depth class go together into one multi-query launch.
+// dispatch).  By default each request is one launch, chosen highest response
+// ratio first ((wait + S) / S, S the query's last service time); with
+// max_batch > 1, the waiting count requests of the oldest one's stack-depth
+// class go together into one multi-query launch.
 // An enumerate request runs alone once its
 // row buffer fits in the enumeration arena (admission by device memory).
 // Completion is detected with an event per launch, the counts and scheduler
@@ -53,6 +54,9 @@ struct Request {
   bool finished = false;
   tm_status status = TM_OK;
   int batch = 0;
+  std::string key;  // the query (motif, delta, cap): its service times so far are the dispatcher's estimate
+  bool limited = false;  // running under the first-run time slice
+  bool probed = false;   // its time slice ran out once: the next run is a full one
   uint64_t count = 0;
   std::vector<int32_t> rows;
   size_t arena_off = 0, arena_len = 0;
@@ -136,6 +140,16 @@ struct tm_server {
   Arena arena;
   Lane lane[TM_SLOTS];
   uint64_t n_launch = 0, n_count = 0, n_enum = 0, max_seen = 0;
+  // service time (launch to result, ms) last seen per query: one request per
+  // launch is dispatched highest response ratio first, (wait + S) / S, with S
+  // this estimate (1 ms for a query not seen yet, so it runs soon and is learnt)
+  std::unordered_map<std::string, double> service_ms;
+  // the first run of a count query not seen yet gets this time limit (the
+  // kernel's own deadline); one that runs out is re-queued for a full run
+  // with an estimate of kHeavyMs: a heavy query costs at most one slice more
+  // and no longer holds a launch slot ahead of light ones it could not know
+  static constexpr int kSliceMs = 25;
+  static constexpr double kHeavyMs = 1000.0;
 };
 
 namespace {
@@ -150,11 +164,43 @@ bool launch_next(tm_server *s, int k) {
   Lane &ln = s->lane[k];
   Slot &sl = s->g->slot[k];
   if (s->waiting.empty()) return false;
-  // the oldest waiting request picks the kind and class of the launch;
-  // an enumerate request that does not fit the arena yet is passed over
+  // one request per launch: the highest response ratio first; class
+  // batches: the oldest waiting request picks the kind and class of the
+  // launch.  An enumerate request that does not fit the arena yet is passed
+  // over either way.
   std::vector<uint64_t> pick;
   bool enumerate = false;
-  for (auto it = s->waiting.begin(); it != s->waiting.end(); ++it) {
+  if (s->max_batch == 1) {
+    // one request per launch: highest response ratio next, so light requests
+    // do not queue behind heavy ones and heavy ones still age to the front
+    const Clock::time_point now = Clock::now();
+    std::vector<std::pair<double, std::deque<uint64_t>::iterator>> order;
+    order.reserve(s->waiting.size());
+    for (auto it = s->waiting.begin(); it != s->waiting.end(); ++it) {
+      const Request &r = *s->reqs[*it];
+      auto c = s->service_ms.find(r.key);
+      const double S = c == s->service_ms.end() ? 1.0 : std::max(c->second, 0.01);
+      order.push_back({(ms_between(r.submit, now) + S) / S, it});
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    for (auto &o : order) {
+      Request &r = *s->reqs[*o.second];
+      if (r.cap >= 0) {
+        size_t off = 0;
+        const size_t len = sizeof(int32_t) * (size_t)std::max<int64_t>(r.cap, 1) * r.M;
+        if (!s->arena.alloc(len, &off)) continue;  // waits for running enumerations to free the arena
+        r.arena_off = off;
+        r.arena_len = len;
+        enumerate = true;
+      } else {
+        r.limited = !r.probed && s->service_ms.find(r.key) == s->service_ms.end();
+      }
+      pick.push_back(*o.second);
+      s->waiting.erase(o.second);
+      break;
+    }
+  }
+  for (auto it = s->waiting.begin(); pick.empty() && s->max_batch > 1 && it != s->waiting.end(); ++it) {
     Request &r = *s->reqs[*it];
     if (r.cap >= 0) {
       size_t off = 0;
@@ -213,6 +259,12 @@ bool launch_next(tm_server *s, int k) {
   const tm_tuning *tune = s->own_tune ? &s->tune
                                       : ((cls >= 0 && cls < TM_NCLASSES && s->g->class_set[cls]) ? &s->g->class_tune[cls]
                                                                                                 : nullptr);
+  tm_tuning sliced;
+  if (n == 1 && s->reqs[pick[0]]->limited) {
+    if (tune) sliced = *tune; else memset(&sliced, 0, sizeof(sliced));
+    if (sliced.time_limit_ms <= 0 || sliced.time_limit_ms > tm_server::kSliceMs) sliced.time_limit_ms = tm_server::kSliceMs;
+    tune = &sliced;
+  }
   if (st == TM_OK) st = launch(s->g, sl, 0, n, enumerate, tune, false, !enumerate && n > 1);
   hipError_t e = hipSuccess;
   if (st == TM_OK) e = hipMemcpyAsync(ln.h_sched, sl.s.d_sched, sizeof(tm_sched), hipMemcpyDeviceToHost, sl.stream);
@@ -248,6 +300,15 @@ void complete(tm_server *s, int k) {
   const tm_status st = collect(s->g, sl, *ln.h_sched);
   for (size_t i = 0; i < ln.ids.size(); i++) {
     Request &r = *s->reqs[ln.ids[i]];
+    if (r.limited && st == TM_ERR_TIMEOUT) {
+      // its first-run slice ran out: back to the queue for a full run
+      r.limited = false;
+      r.probed = true;
+      s->service_ms[r.key] = tm_server::kHeavyMs;
+      s->waiting.push_back(ln.ids[i]);
+      continue;
+    }
+    r.limited = false;
     r.status = st;
     r.count = ln.h_counts[i];
     if (ln.enumerate) {
@@ -282,6 +343,7 @@ void complete(tm_server *s, int k) {
     }
     r.done = Clock::now();
     r.finished = true;
+    if (ln.ids.size() == 1 && r.status == TM_OK) s->service_ms[r.key] = ms_between(r.start, r.done);
   }
   s->running -= (int)ln.ids.size();
   ln.ids.clear();
@@ -411,6 +473,13 @@ tm_status tm_server_submit(tm_server *s, int32_t num_edges, const int32_t *msrc,
   r->delta = delta;
   r->cap = enum_cap;
   r->cls = depth_class(md.M, md.n);
+  {
+    int32_t k[2 + 2 * TM_MAX_MOTIF_EDGES] = {num_edges, delta};
+    memcpy(k + 2, msrc, sizeof(int32_t) * num_edges);
+    memcpy(k + 2 + num_edges, mdst, sizeof(int32_t) * num_edges);
+    r->key.assign((const char *)k, sizeof(int32_t) * (2 + 2 * num_edges));
+    r->key.append((const char *)&enum_cap, sizeof(enum_cap));
+  }
   if (enum_cap >= 0) {
     // the piece the arena would hand out (rounded up to 256 bytes) must fit
     // an empty arena, and its size must not overflow

@@ -332,12 +332,14 @@ tm_status tm_batch_free(tm_batch *b);
  *
  * Any host thread submits count or enumerate requests.  A dispatcher thread
  * forms launches from whatever is queued whenever one of the server's launch
- * slots is free (continuous dispatch), oldest request first: with max_batch
- * 1 (the default) each count request is one single-query launch, so a light
- * request overtakes a heavy one on another slot instead of waiting for it
- * inside a batch; with max_batch > 1 the queued count requests of the oldest
- * request's stack-depth class go together into one multi-query launch (at
- * most max_batch); an enumerate request runs alone and is admitted only
+ * slots is free (continuous dispatch).  With max_batch 1 (the default) each
+ * request is one single-query launch, the waiting request with the highest
+ * response ratio (wait + S) / S first, S the service time of that query's
+ * last run (1 ms for a query not seen yet): light requests overtake heavy
+ * ones and heavy ones age to the front.  With max_batch > 1 the queued count
+ * requests of the oldest request's stack-depth class go together into one
+ * multi-query launch (at most max_batch).  An enumerate request runs alone
+ * and is admitted only
  * once its row buffer (enum_cap * num_edges int32) fits in the enumeration
  * arena.  Launches on different slots run concurrently on the device, so
  * requests that arrive while others run overlap with them.  Each result is
