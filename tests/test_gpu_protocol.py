@@ -114,7 +114,7 @@ def test_donation_ring_wraps(pl8m):
     from everest_amd.gen import queries
     g, dg = pl8m
     mot = queries.as_motif([(0, 1), (0, 2), (0, 3), (0, 1), (0, 2)])
-    beg, end = 0, 2_000_000
+    beg, end = 0, 3_000_000
     say("pl8m ring wrap", mot, "roots", beg, end)
     want = dg.count(mot, 604800, beg, end)
     got = dg.count(mot, 604800, beg, end, tune=abi.tuning(**FORCE))
