@@ -67,6 +67,8 @@ def lib():
         L.tm_device_count.restype = I32
         L.tm_graph_upload.argtypes = [I32, ctypes.POINTER(HostGraph), ctypes.POINTER(P)]
         L.tm_graph_free.argtypes = [P]
+        L.tm_time_order.argtypes = [I32, I64, P, P]
+        L.tm_csr_build.argtypes = [I32, I64, P, P, P, P, P, P, P, P, P, ctypes.POINTER(I32)]
         L.tm_graph_info.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I32), ctypes.POINTER(I32),
                                     ctypes.POINTER(I32)]
         L.tm_count.argtypes = [P, I32, P, P, I32, I32, I32, ctypes.POINTER(Tuning), P]
@@ -124,6 +126,33 @@ def tuning(**kw):
 
 def tuning_dict(t):
     return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
+
+
+def time_order(T, device=0):
+    """Stable order of positions by time, computed on the GPU (tm_time_order)."""
+    T = np.ascontiguousarray(T, dtype=np.int32)
+    out = np.zeros(T.size, dtype=np.int32)
+    check(lib().tm_time_order(int(device), ctypes.c_int64(T.size), _p(T), _p(out)))
+    return out
+
+
+def csr_build(U, V, T, device=0):
+    """The reference's graph arrays of a time-sorted edge list, built on the
+    GPU (tm_csr_build): dict with Eg (n x 3), outV, inV, outR, inR, rename
+    (raw id of each new id), V (vertex count)."""
+    U = np.ascontiguousarray(U, dtype=np.int32)
+    V = np.ascontiguousarray(V, dtype=np.int32)
+    T = np.ascontiguousarray(T, dtype=np.int32)
+    n = U.size
+    Eg = np.zeros(3 * n, dtype=np.int32)
+    outV, inV = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+    outR, inR, raw = (np.zeros(2 * n + 1, dtype=np.int32) for _ in range(3))
+    nv = ctypes.c_int32()
+    check(lib().tm_csr_build(int(device), ctypes.c_int64(n), _p(U), _p(V), _p(T), _p(Eg), _p(outV), _p(inV),
+                             _p(outR), _p(inR), _p(raw), ctypes.byref(nv)))
+    k = nv.value
+    return dict(Eg=Eg.reshape(-1, 3), outV=outV, inV=inV, outR=outR[:k + 1].copy(), inR=inR[:k + 1].copy(),
+                rename=raw[:k].copy(), V=k)
 
 
 def shape_class(edges):

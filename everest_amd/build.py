@@ -48,7 +48,7 @@ def ext_suffix():
     return sysconfig.get_config_var("EXT_SUFFIX") or ".so"
 
 
-HIP_UNITS = ("tm_abi.hip", "tm_sweep.hip", "tm_server.hip", "tm_batch.hip", "tm_noalloc.hip", "tm_kv_n4_single.hip", "tm_kv_n4_multi.hip",
+HIP_UNITS = ("tm_abi.hip", "tm_sweep.hip", "tm_build.hip", "tm_server.hip", "tm_batch.hip", "tm_noalloc.hip", "tm_kv_n4_single.hip", "tm_kv_n4_multi.hip",
              "tm_kv_n4_aux.hip", "tm_kv_n8_single.hip", "tm_kv_n8_multi.hip", "tm_kv_n8_aux.hip")
 
 

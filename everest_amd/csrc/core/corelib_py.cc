@@ -104,6 +104,8 @@ PYBIND11_MODULE(corelib, m) {
       });
 
   d.def("divideList", &data::divideList);
+  d.def("setBuildDevice", &data::setBuildDevice, "GPU for the time sort and CSR build (-1 host, -2 auto)");
+  d.def("buildDevice", &data::buildDevice);
   d.def("removeMinor", &data::removeMinor);
   d.def("includeMinor", &data::includeMinor);
 

@@ -23,6 +23,7 @@
 #include <thread>
 #include <unordered_map>
 
+#include "tm_abi.h"
 #include "everest/data.h"
 
 namespace corelib::util {
@@ -200,8 +201,31 @@ void parse_range(const char *b, const char *e, std::vector<int> &U, std::vector<
   }
 }
 
-// Stable permutation sorting by T (LSD radix sort, 2 x 16-bit digits).
+}  // namespace
+
+static std::atomic<int> g_build_device{-2};
+void setBuildDevice(int device) { g_build_device = device; }
+int buildDevice() {
+  const int d = g_build_device;
+  if (d != -2) return d;
+  return tm_device_count() > 0 ? 0 : -1;
+}
+
+namespace {
+
+[[noreturn]] void build_failed(const char *what) {
+  throw std::runtime_error(std::string(what) + " on the GPU failed: " + tm_last_error());
+}
+
+// Stable permutation sorting by T: on the build device (radix sort of
+// (time, position) pairs), else on the host (LSD radix sort, 2 x 16-bit digits).
 std::vector<int> stable_time_order(const int *T, size_t n) {
+  const int dev = buildDevice();
+  if (dev >= 0) {
+    std::vector<int> ids(n);
+    if (tm_time_order(dev, (int64_t)n, T, ids.data()) != TM_OK) build_failed("stable time sort");
+    return ids;
+  }
   std::vector<int> a(n), b(n);
   std::iota(a.begin(), a.end(), 0);
   std::vector<size_t> cnt(65537);
@@ -476,6 +500,8 @@ HostGraphData GraphDataLoader::createGraphData(size_t beg, size_t end, bool cach
   const int *U = (const int *)UPtr_->data() + beg;
   const int *V = (const int *)VPtr_->data() + beg;
   const int *T = (const int *)TPtr_->data() + beg;
+  const int dev = buildDevice();
+  if (dev >= 0) return createGraphDataOnDevice(dev, beg, end, cache);
 
   // ---- rename by first occurrence (u before v of the same edge)
   std::vector<int> Un(n), Vn(n), raw;
@@ -551,6 +577,62 @@ HostGraphData GraphDataLoader::createGraphData(size_t beg, size_t end, bool cach
   csr(Un, (int *)res.outEdgesR_h->data(), (int *)res.outEdgesV_h->data());
   csr(Vn, (int *)res.inEdgesR_h->data(), (int *)res.inEdgesV_h->data());
   // ---- features and Euid
+  const int *FE = (const int *)FEPtr_->data();
+  if (n) memcpy(res.edgeFeatures->data(), FE + beg, n * sizeof(int));
+  const int *FVM = (const int *)FVMPtr_->data();
+  const int *FVV = (const int *)FVVPtr_->data();
+  int *VF = (int *)res.verticesFeatures->data();
+  for (size_t k = 0; k < nv; k++) {
+    const int *it = std::lower_bound(FVM, FVM + verticesFeatureLength_, raw[k]);
+    size_t idx = (size_t)(it - FVM);
+    VF[k] = idx < verticesFeatureLength_ ? FVV[idx] : 0;
+  }
+  const int *IDS = (const int *)EuidPtr_->data();
+  if (n) memcpy(euid->data(), IDS + beg, n * sizeof(int));
+  res.Euid_h = std::move(euid);
+  return res;
+}
+
+// The same arrays built on a GPU (tm_csr_build: stable radix sorts for the
+// renaming and both CSR indexes), then placed in the graph's host buffers.
+HostGraphData GraphDataLoader::createGraphDataOnDevice(int dev, size_t beg, size_t end, bool cache) const {
+  const size_t n = end - beg;
+  const int *U = (const int *)UPtr_->data() + beg;
+  const int *V = (const int *)VPtr_->data() + beg;
+  const int *T = (const int *)TPtr_->data() + beg;
+  std::vector<int> Eg3(3 * n), outV(n), inV(n), outR(2 * n + 1), inR(2 * n + 1), raw(2 * n + 1);
+  int32_t nv32 = 0;
+  if (tm_csr_build(dev, (int64_t)n, U, V, T, Eg3.data(), outV.data(), inV.data(), outR.data(), inR.data(), raw.data(),
+                   &nv32) != TM_OK)
+    build_failed("graph build");
+  const size_t nv = (size_t)nv32;
+
+  HostGraphData res;
+  res.name = cacheGraphDirPath(beg, end).filename().string();
+  res.beg = beg;
+  res.end = end;
+  res.numEdges = n;
+  res.numVertices = nv;
+  res.totalBytes = calcTotalBytes(n, nv);
+  res.uid = g_graph_uid.fetch_add(1);
+  auto cpath = cacheGraphDirPath(beg, end);
+  auto sink = [&](const char *file, size_t bytes) -> SinkMemory * {
+    if (cache) return new MappedFileSinkMemory(cpath / file, bytes);
+    return new VectorSinkMemory(bytes);
+  };
+  if (cache) std::filesystem::create_directories(cpath);
+  auto put = [&](std::unique_ptr<SinkMemory> &dst, const char *file, const int *src, size_t count) {
+    dst.reset(sink(file, count * sizeof(int)));
+    if (count) util::memcpyPar(dst->data(), src, count * sizeof(int));
+  };
+  put(res.Eg_h, "Eg", Eg3.data(), 3 * n);
+  put(res.inEdgesV_h, "inEdgesV", inV.data(), n);
+  put(res.outEdgesV_h, "outEdgesV", outV.data(), n);
+  put(res.inEdgesR_h, "inEdgesR", inR.data(), nv + 1);
+  put(res.outEdgesR_h, "outEdgesR", outR.data(), nv + 1);
+  res.edgeFeatures.reset(sink("edgeFeatures", n * sizeof(int)));
+  res.verticesFeatures.reset(sink("verticesFeatures", nv * sizeof(int)));
+  std::unique_ptr<SinkMemory> euid(sink("Euid", n * sizeof(int)));  // last: marks a complete cache directory
   const int *FE = (const int *)FEPtr_->data();
   if (n) memcpy(res.edgeFeatures->data(), FE + beg, n * sizeof(int));
   const int *FVM = (const int *)FVMPtr_->data();

@@ -211,6 +211,13 @@ struct SubPartition {
 typedef std::vector<SubPartition> SubPartitionList;
 SubPartitionList divideList(const EdgeListLoader &eloader, int delta, int num);
 
+// Where the stable time sort (EdgeListLoader) and the renaming + CSR build
+// (GraphDataLoader::createGraphData) run: a GPU index (tm_time_order /
+// tm_csr_build), -1 for the host, -2 (the default) for GPU 0 when one is
+// visible and the host otherwise.  The arrays are the same either way.
+void setBuildDevice(int device);
+int buildDevice();
+
 typedef std::vector<HostGraphData> HostGraphDataList;
 void removeMinor(HostGraphDataList &complete);
 void includeMinor(HostGraphDataList &major, HostGraphDataList &minor);
@@ -225,6 +232,7 @@ class GraphDataLoader : public CacheDirectoryAccess {
   size_t edgeListLength_, verticesFeatureLength_;
 
   HostGraphData loadGraphData(size_t beg, size_t end) const;
+  HostGraphData createGraphDataOnDevice(int device, size_t beg, size_t end, bool cache) const;
   bool isCached(size_t beg, size_t end) const;
 
  public:

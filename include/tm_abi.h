@@ -103,6 +103,28 @@ const char *tm_last_error(void);
 int32_t tm_device_count(void);
 
 /*
+ * Graph arrays built on the device (no graph handle needed).
+ *
+ * tm_time_order: `order` (n positions) = the stable order of the edges by
+ * time T, ties in input order.  Replaces the host-side ranges::stable_sort of
+ * EdgeListLoader::createEdgeListCache (src/core/cpu/data.cc:246-248).
+ *
+ * tm_csr_build: for a time-sorted edge list (U, V, T of n edges), the
+ * reference's graph arrays: vertices renamed by first appearance (a source
+ * before the destination of the same edge), Eg (3n ints: u, v, t), the
+ * positions grouped by renamed source / destination in ascending order
+ * (outEdgesV / inEdgesV, n each) with their offsets (outEdgesR / inEdgesR,
+ * numVertices + 1 each), and vertexIds[k] = the raw id renamed to k.  The
+ * caller sizes outEdgesR, inEdgesR and vertexIds for 2n + 1 entries (at most
+ * 2n vertices).  Replaces GraphDataLoader::createGraphData's renaming and CSR
+ * build (src/core/gpu/data.cu:124-348).
+ */
+tm_status tm_time_order(int32_t device, int64_t n, const int32_t *T, int32_t *order);
+tm_status tm_csr_build(int32_t device, int64_t n, const int32_t *U, const int32_t *V, const int32_t *T, int32_t *Eg,
+                       int32_t *outEdgesV, int32_t *inEdgesV, int32_t *outEdgesR, int32_t *inEdgesR,
+                       int32_t *vertexIds, int32_t *numVertices);
+
+/*
  * Upload a host graph to `device` and build the device-side indexes
  * (packed times, neighbour arrays, per-pair index, time buckets).
  * Replaces GraphDataManager::alloc + moveAsync + waitMove
