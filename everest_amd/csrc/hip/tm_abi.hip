@@ -540,6 +540,8 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   memset(&a, 0, sizeof(a));
   a.g = g->view;
   a.gv = g->d_view;
+  a.edgeFeature = g->edge_features;
+  a.vertexFeature = g->vertex_features;
   a.queries = s.d_queries + q0;
   a.qtab = s.d_qtab + 4 * (size_t)q0;
   a.block_prefix = s.d_prefix + q0;

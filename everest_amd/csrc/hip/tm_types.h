@@ -201,4 +201,8 @@ typedef struct {
   // variables ends up in per-lane scratch memory once the compiler selects
   // between two of its fields' addresses (sunk loads of out- vs in-lists).
   const tm_graph_view *gv;
+  // labels set by tm_graph_set_features (null when none): per edge position
+  // and per vertex, read by kernels of queries with label constraints
+  const int *edgeFeature;
+  const int *vertexFeature;
 } tm_kernel_args;

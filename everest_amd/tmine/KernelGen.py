@@ -8,6 +8,7 @@ time arrays per edge), the per-query options baked into its launch
 parameters, and the reference worker entry point getWorker.
 """
 import os
+import re
 from string import Template
 
 from . import locations
@@ -123,15 +124,89 @@ def motif_tables(edges):
     return dict(M=M, N=n, kind=kind, a=A, b=B, mapped=mapped, smode=smode, tmode=tmode, xmask=xmask)
 
 
+def _c_expr(expr):
+    """A constraint expression as the reference pastes it into device code
+    (Query.py:229-280): a C expression over `f` (a label), or for the filter
+    over eK / nK; only checked to be one expression without statements."""
+    expr = str(expr).strip()
+    if not expr or any(c in expr for c in ";{}#\\\n"):
+        raise ValueError("constraint expression must be one C expression: %r" % expr)
+    return expr
+
+
+def constraint_bodies(cons, M, N):
+    """Device-function bodies of the constrained worker (templates/worker_cons.hip.tmpl):
+    label predicates per motif node / edge (a `node` or `edge` key applies to
+    all, `nodeK` / `edgeK` to motif node / edge K and takes precedence), the
+    window of each edge after the previous one (`temporal`: gaps between
+    consecutive motif edges, KernelGen.py:467-477) and the filter over a
+    complete match (`filter`, with eK / nK the labels of matched edge / node
+    K, KernelGen.py:530-534 and Query.py:84-102)."""
+    c = cons.cons
+    def per(prefix, count):
+        out = {}
+        for i in range(count):
+            if "%s%d" % (prefix, i) in c:
+                out[i] = _c_expr(c["%s%d" % (prefix, i)])
+            elif prefix in c:
+                out[i] = _c_expr(c[prefix])
+        return out
+    nodes, edges = per("node", N), per("edge", M)
+    def body(preds, arr, idx, var):
+        if not preds:
+            return "    (void)%s; (void)%s; (void)%s;\n    return true;" % (var, arr, idx)
+        lines = ["    const int f = %s[%s];" % (arr, idx), "    switch (%s) {" % var]
+        for i, e in sorted(preds.items()):
+            lines.append("      case %d: return (%s);" % (i, e))
+        lines += ["      default: return true;", "    }"]
+        return "\n".join(lines)
+    node_body = body(nodes, "VF", "node", "i")
+    edge_body = body(edges, "EF", "p", "j")
+    if "temporal" in c:
+        gaps = [int(x) for x in c["temporal"]]
+        if len(gaps) < M - 1:
+            raise ValueError("temporal needs %d gaps (one per pair of consecutive motif edges)" % (M - 1))
+        window_body = ("    constexpr int tw[TM_MAXE] = {%s};\n"
+                       "    (void)t0;\n"
+                       "    const long long w = tmk::pick(tw, j);\n"
+                       "    if (j == 0) return tj + w;\n"
+                       "    return tj + w < tmax ? tj + w : tmax;"
+                       % ", ".join(str(x) for x in (gaps + [0] * MAXE)[:MAXE]))
+    else:
+        window_body = "    (void)j; (void)tj; (void)t0;\n    return tmax;"
+    filt = _c_expr(c["filter"]) if "filter" in c else None
+    uses_e = uses_n = False
+    if filt:
+        uses_e = re.search(r"\be\d+\b", filt) is not None
+        uses_n = re.search(r"\bn\d+\b", filt) is not None
+        for k in [int(x) for x in re.findall(r"\be(\d+)\b", filt)]:
+            if k >= M:
+                raise ValueError("filter names e%d of a %d-edge motif" % (k, M))
+        for k in [int(x) for x in re.findall(r"\bn(\d+)\b", filt)]:
+            if k >= N:
+                raise ValueError("filter names n%d of a %d-node motif" % (k, N))
+        expr = re.sub(r"\be(\d+)\b", r"EF[pos[\1]]", filt)
+        expr = re.sub(r"\bn(\d+)\b", r"VF[m[\1]]", expr)
+        filter_body = "    (void)EF; (void)VF; (void)pos; (void)m;\n    return (%s);" % expr
+    else:
+        filter_body = "    (void)EF; (void)VF; (void)pos; (void)m;\n    return true;"
+    return dict(node_body=node_body, edge_body=edge_body, window_body=window_body, filter_body=filter_body,
+                need_ef="true" if (edges or uses_e) else "false", need_vf="true" if (nodes or uses_n) else "false")
+
+
 class KernelGen:
     def __init__(self, pathToMotif, query, tuning=None):
         self.query = query
         self.options = query.options()
         self.cons = query.constraints()
         feats = self.cons.features()
-        if feats:
-            raise NotImplementedError("constraint features %s are not supported by this build "
-                                      "(the delta-window query path is)" % feats)
+        if "antiEdge" in feats:
+            raise NotImplementedError("antiEdge constraints are not supported by this build "
+                                      "(label predicates, temporal windows and filters are)")
+        self.constrained = bool(feats)
+        if self.constrained and self.options.enumerate():
+            raise NotImplementedError("enumeration under constraint features is not supported by this build "
+                                      "(counting is)")
         if self.options.multigpu():
             raise NotImplementedError("this build drives one GPU (options.gpu = %d)" % self.options.gpu())
         self.pathToMotif = pathToMotif
@@ -157,9 +232,18 @@ class KernelGen:
 
     def generate(self):
         tb = self.tables
-        with open(os.path.join(locations.templateRoot(), "worker.hip.tmpl")) as f:
+        name = "worker_cons.hip.tmpl" if self.constrained else "worker.hip.tmpl"
+        with open(os.path.join(locations.templateRoot(), name)) as f:
             tmpl = Template(f.read())
         join = lambda xs: ", ".join(str(int(x)) for x in xs)
+        if self.constrained:
+            subs = dict(motif_name=os.path.basename(self.pathToMotif), M=tb["M"], N=tb["N"],
+                        options=", ".join("%s=%s" % kv for kv in sorted(self.options.opts.items())),
+                        cons_text=" ".join(str(self.cons.cons).split()),
+                        src=join(a for a, _ in self.edges), dst=join(b for _, b in self.edges),
+                        kind=join(tb["kind"]), a=join(tb["a"]), b=join(tb["b"]), mapped=join(tb["mapped"]))
+            subs.update(constraint_bodies(self.cons, tb["M"], tb["N"]))
+            return tmpl.substitute(subs)
         subs = dict(motif_name=os.path.basename(self.pathToMotif), M=tb["M"], N=tb["N"],
                     options=", ".join("%s=%s" % kv for kv in sorted(self.options.opts.items())),
                     src=join(a for a, _ in self.edges), dst=join(b for _, b in self.edges),

@@ -7,9 +7,11 @@ A query file holds:
     options: {gpu, warpCentric, warpBalancing, tailExpansion, enumerate}
 
 Options default to gpu=1 and warpCentric/warpBalancing/tailExpansion=True
-(Query.py:6-12).  The constraint features of the reference (temporal,
-antiEdge, filter, node*/edge* label predicates) are parsed but rejected by
-KernelGen: this build implements the delta-window query path.
+(Query.py:6-12).  Constraint features: label predicates (`node`, `nodeK`,
+`edge`, `edgeK`: C expressions over the label `f`), temporal gaps
+(`temporal`) and a filter over the matched labels (`filter`: eK / nK) are
+compiled into a constrained worker by KernelGen (counting); antiEdge, and
+enumeration under constraints, are rejected.
 """
 import os
 
@@ -86,7 +88,8 @@ class Constraints:
 
     def genConsTemplates(self):
         """The reference writes constraint snippets here (Query.py:136-166);
-        the delta-window path needs none."""
+        this build emits them into the worker itself (KernelGen
+        constraint_bodies)."""
         return None
 
 
