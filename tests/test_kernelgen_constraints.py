@@ -1,0 +1,52 @@
+"""KernelGen for constraint features (no GPU): the generated constrained
+worker holds the predicates, gaps and filter as device code and compiles
+with hipcc for gfx950; anti-edges and enumeration under constraints are
+rejected; malformed expressions are refused."""
+import os
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _query(tmp, cons_lines, opts=""):
+    p = os.path.join(tmp, "q.yaml")
+    with open(p, "w") as f:
+        f.write("pattern: hub5.txt\ngraph: g.txt\nconstraints:\n  delta: 100\n%s\noptions:\n  gpu: 1\n%s" % (
+            "".join("  %s\n" % x for x in cons_lines), opts))
+    return p
+
+
+def test_constrained_worker_generates_and_compiles(monkeypatch):
+    from everest_amd.tmine import KernelGen, Query, utils
+    monkeypatch.setenv("EVEREST_INPUTS", os.path.join(ROOT, "inputs"))
+    with tempfile.TemporaryDirectory() as tmp:
+        q = Query.Query(_query(tmp, ['node: "f % 2 == 0"', 'edge3: "f > 7"', "temporal: [5, 6, 7, 8]",
+                                     'filter: "e0 < e4 && n2 != n3"']))
+        code = KernelGen.KernelGen(q.pattern(), q).generate()
+        assert "tmc::cons_kernel<CSpec>" in code
+        assert "case 3: return (f % 2 == 0);" in code and "case 3: return (f > 7);" in code
+        assert "{5, 6, 7, 8, 0, 0, 0, 0}" in code
+        assert "return (EF[pos[0]] < EF[pos[4]] && VF[m[2]] != VF[m[3]]);" in code
+        src = os.path.join(tmp, "w.hip")
+        with open(src, "w") as f:
+            f.write(code)
+        utils.compileModule(src, os.path.join(tmp, "w.so"))
+        assert os.path.getsize(os.path.join(tmp, "w.so")) > 0
+
+
+def test_unsupported_constraint_forms(monkeypatch):
+    from everest_amd.tmine import KernelGen, Query
+    monkeypatch.setenv("EVEREST_INPUTS", os.path.join(ROOT, "inputs"))
+    with tempfile.TemporaryDirectory() as tmp:
+        q = Query.Query(_query(tmp, ["antiEdge: [false, false, true, false, false]"]))
+        with pytest.raises(NotImplementedError):
+            KernelGen.KernelGen(q.pattern(), q)
+        q = Query.Query(_query(tmp, ['node: "f > 1"'], "  enumerate: 10\n"))
+        with pytest.raises(NotImplementedError):
+            KernelGen.KernelGen(q.pattern(), q)
+        for bad in ['node: "f > 1; return 0"', 'filter: "e9 > 1"', "temporal: [1, 2]"]:
+            q = Query.Query(_query(tmp, [bad]))
+            with pytest.raises(ValueError):
+                KernelGen.KernelGen(q.pattern(), q).generate()
