@@ -2,7 +2,9 @@
 // motif nodes and edges, fine-grained temporal windows between consecutive
 // motif edges, and a filter over the matched edges' and nodes' labels
 // (reference: system/tmine/Query.py:136-308, system/tmine/KernelGen.py:
-// 467-477,530-534, system/templates/constraints/*).
+// 467-477,530-534, system/templates/constraints/*) and anti-edges
+// (system/templates/common/lookAtMatchRes.snp:1-20, findMatch.snp:1-14,
+// nextLevel.snp:1-6, system/tmine/Query.py:39-77, KernelGen.py:479-497).
 //
 // A predicate can reject any candidate, so none of the mining kernel's
 // closed forms (range-arithmetic leaves, two-edge tails, window sweep)
@@ -19,9 +21,16 @@
 //   edge_ok(j, EF, p)          label predicate of motif edge j (graph position p)
 //   window(j, t_j, t0, tmax)   window end for edge j + 1 after edge j at t_j
 //   filter(EF, VF, pos, m)     over a complete match
-// The window of edge j + 1 is min(t_j + temporal[j], t0 + delta) for j >= 1
+//   anti(j), prev(j)           edge j is an anti edge; the nearest earlier
+//                              edge that is not
+// The window of edge j + 1 is min(t_p + temporal[j], t0 + delta) for j >= 1
 // and t0 + temporal[0] for edge 1 (KernelGen.py:467-477), t0 + delta
-// without a temporal list.
+// without a temporal list, where p = prev(j + 1) (j itself without anti
+// edges).  An anti edge j is a check, not a match position: the partial
+// match survives it when no candidate of edge j (both endpoints mapped; the
+// edge's label predicate applied) lies after pos[prev(j)] within its window;
+// the next edge's list and window start from pos[prev(j)] as well, and the
+// search backtracks to the nearest earlier non-anti edge.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -57,8 +66,9 @@ __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
     while (j >= 1) {
       if (open) {
         open = false;
-        tl[j] = CS::window(j - 1, j == 1 ? t0 : (long long)g.T[pos[j - 1]], t0, tmax);
-        const int p = pos[j - 1];
+        const int pj = CS::prev(j);
+        const int p = pos[pj];
+        tl[j] = CS::window(j - 1, pj == 0 ? t0 : (long long)g.T[p], t0, tmax);
         const int k = CS::kind(j);
         if (k == TM_K_NEW_DST) {
           const int x = m[CS::a(j)];
@@ -77,8 +87,28 @@ __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
           lim[j] = s1;
         }
       }
+      if (CS::anti(j)) {  // an anti edge: any candidate within the window rejects the partial match
+        bool hit = false;
+        while (!hit && cur[j] < lim[j]) {
+          const int p = g.outPairPos[cur[j]++];
+          if ((long long)g.T[p] > tl[j]) break;
+          hit = CS::edge_ok(j, EF, p);
+        }
+        const int pj = CS::prev(j);
+        if (!hit) {
+          pos[j] = pos[pj];
+          if (j < M - 1) {
+            j++;
+            open = true;
+            continue;
+          }
+          if (CS::filter(EF, VF, pos, m)) cnt++;
+        }
+        j = pj;
+        continue;
+      }
       if (cur[j] >= lim[j]) {
-        j--;
+        j = CS::prev(j);
         continue;
       }
       const int s = cur[j]++;

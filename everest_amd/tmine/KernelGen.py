@@ -134,7 +134,43 @@ def _c_expr(expr):
     return expr
 
 
-def constraint_bodies(cons, M, N):
+def anti_tables(cons, tables):
+    """Anti-edge flags and, per motif edge, the nearest earlier edge that is
+    not an anti edge (Query.py:39-77: the reference's antiEdge,
+    antiEdgeOffset and antiEdgePrev arrays).  Defined here for anti edges
+    whose two endpoints are mapped by earlier edges, not directly after the
+    root unless last: for an anti edge with a new endpoint the reference maps
+    that node to the last candidate it looked at, and after an anti edge at
+    position 1 it starts the next edge's list one past the graph's last edge
+    (system/tmine/KernelGen.py:490-497), so neither has a count to match."""
+    M = tables["M"]
+    c = cons.cons
+    if "antiEdge" not in c:
+        return [False] * M, list(range(-1, M - 1)) if M else []
+    raw = c["antiEdge"]
+    if not isinstance(raw, (list, tuple)) or len(raw) != M:
+        raise ValueError("antiEdge needs one flag per motif edge (%d)" % M)
+    anti = []
+    for x in raw:
+        if isinstance(x, str):
+            if x.strip().lower() not in ("true", "false"):
+                raise ValueError("antiEdge flags are true / false: %r" % (x,))
+            x = x.strip().lower() == "true"
+        anti.append(bool(x))
+    if anti[0]:
+        raise ValueError("the root edge cannot be an anti edge")
+    for j in range(1, M):
+        if anti[j] and tables["kind"][j] != 2:
+            raise NotImplementedError("anti edge %d introduces a motif node; anti edges between nodes that "
+                                      "earlier edges map are supported" % j)
+    if M > 2 and anti[1]:
+        raise NotImplementedError("an anti edge directly after the root edge is supported as the motif's "
+                                  "last edge only")
+    prev = [-1] + [max(i for i in range(j) if not anti[i]) for j in range(1, M)]
+    return anti, prev
+
+
+def constraint_bodies(cons, M, N, anti=None):
     """Device-function bodies of the constrained worker (templates/worker_cons.hip.tmpl):
     label predicates per motif node / edge (a `node` or `edge` key applies to
     all, `nodeK` / `edgeK` to motif node / edge K and takes precedence), the
@@ -182,6 +218,9 @@ def constraint_bodies(cons, M, N):
         for k in [int(x) for x in re.findall(r"\be(\d+)\b", filt)]:
             if k >= M:
                 raise ValueError("filter names e%d of a %d-edge motif" % (k, M))
+            if anti and anti[k]:
+                # the reference reads the label one past the graph's last edge here
+                raise ValueError("filter names e%d, an anti edge, which matches no graph edge" % k)
         for k in [int(x) for x in re.findall(r"\bn(\d+)\b", filt)]:
             if k >= N:
                 raise ValueError("filter names n%d of a %d-node motif" % (k, N))
@@ -200,9 +239,6 @@ class KernelGen:
         self.options = query.options()
         self.cons = query.constraints()
         feats = self.cons.features()
-        if "antiEdge" in feats:
-            raise NotImplementedError("antiEdge constraints are not supported by this build "
-                                      "(label predicates, temporal windows and filters are)")
         self.constrained = bool(feats)
         if self.constrained and self.options.enumerate():
             raise NotImplementedError("enumeration under constraint features is not supported by this build "
@@ -212,6 +248,7 @@ class KernelGen:
         self.pathToMotif = pathToMotif
         self.edges = motif_edges(pathToMotif)
         self.tables = motif_tables(self.edges)
+        self.anti, self.prev = anti_tables(self.cons, self.tables)
         self.enumerate = self.options.enumerate()
         self.enum_NUM = self.options.enumNum() if self.enumerate else -1
         t = dict(waves_per_cu=0, claim_blocks=0, scan_max=0, steal_every=0, steal_max=0, steal_min=0,
@@ -242,7 +279,8 @@ class KernelGen:
                         cons_text=" ".join(str(self.cons.cons).split()),
                         src=join(a for a, _ in self.edges), dst=join(b for _, b in self.edges),
                         kind=join(tb["kind"]), a=join(tb["a"]), b=join(tb["b"]), mapped=join(tb["mapped"]))
-            subs.update(constraint_bodies(self.cons, tb["M"], tb["N"]))
+            subs.update(constraint_bodies(self.cons, tb["M"], tb["N"], self.anti))
+            subs.update(anti=join((self.anti + [False] * MAXE)[:MAXE]), prev=join((self.prev + [0] * MAXE)[:MAXE]))
             return tmpl.substitute(subs)
         subs = dict(motif_name=os.path.basename(self.pathToMotif), M=tb["M"], N=tb["N"],
                     options=", ".join("%s=%s" % kv for kv in sorted(self.options.opts.items())),

@@ -9,9 +9,10 @@ A query file holds:
 Options default to gpu=1 and warpCentric/warpBalancing/tailExpansion=True
 (Query.py:6-12).  Constraint features: label predicates (`node`, `nodeK`,
 `edge`, `edgeK`: C expressions over the label `f`), temporal gaps
-(`temporal`) and a filter over the matched labels (`filter`: eK / nK) are
-compiled into a constrained worker by KernelGen (counting); antiEdge, and
-enumeration under constraints, are rejected.
+(`temporal`), a filter over the matched labels (`filter`: eK / nK) and
+anti-edges (`antiEdge`: one flag per motif edge; an anti edge's two
+endpoints mapped by earlier edges) are compiled into a constrained worker by
+KernelGen (counting); enumeration under constraints is rejected.
 """
 import os
 

@@ -34,8 +34,17 @@ def motif_structure(edges):
     return out, n
 
 
+def anti_prev(anti, M):
+    """Per edge j: the nearest earlier non-anti edge (Query.py:64-74)."""
+    anti = list(anti) if anti else [False] * M
+    prev = [0] * M
+    for j in range(1, M):
+        prev[j] = max(i for i in range(j) if not anti[i])
+    return anti, prev
+
+
 def count_constrained(g, edges, delta, EF, VF, node_pred=None, edge_pred=None, temporal=None, filt=None,
-                      beg=0, end=None):
+                      beg=0, end=None, anti=None):
     node_pred, edge_pred = node_pred or {}, edge_pred or {}
     Eg, E = g["Eg"], g["E"]
     T = Eg[:, 2]
@@ -43,6 +52,8 @@ def count_constrained(g, edges, delta, EF, VF, node_pred=None, edge_pred=None, t
     M = len(edges)
     st, N = motif_structure(edges)
     end = E - M + 1 if end is None else end
+    anti, prev = anti_prev(anti, M)
+    assert not anti[0] and all(st[j][0] == 2 for j in range(1, M) if anti[j]), "anti edges join mapped nodes"
     npred = lambda i, x: node_pred[i](int(VF[x])) if i in node_pred else True
     epred = lambda j, p: edge_pred[j](int(EF[p])) if j in edge_pred else True
     total = 0
@@ -74,9 +85,19 @@ def count_constrained(g, edges, delta, EF, VF, node_pred=None, edge_pred=None, t
 
     def dfs(j, m, pos, t0):
         nonlocal total
-        tl = window(j - 1, int(T[pos[-1]]), t0)
+        p = pos[prev[j]]
+        tl = window(j - 1, int(T[p]), t0)
         kind, a, b, nm = st[j]
-        for q, node in candidates(j, m, pos[-1]):
+        if anti[j]:
+            if any(T[q] <= tl and epred(j, q) for q, _ in candidates(j, m, p)):
+                return
+            if j == M - 1:
+                if filt is None or filt([int(EF[x]) for x in pos + [p]], [int(VF[x]) for x in m]):
+                    total += 1
+            else:
+                dfs(j + 1, m, pos + [p], t0)
+            return
+        for q, node in candidates(j, m, p):
             if T[q] > tl:
                 break
             mm = m

@@ -86,3 +86,98 @@ def test_per_item_predicates_temporal_and_filter_are_filtered_rows():
         assert got == ok, (case, edges, d, gaps)
         nz += ok > 0
     assert nz > 10
+
+
+# (motif, anti flags): the reference's example M4-A with its anti edge, an anti
+# edge last, two anti edges in a row, anti edges in the middle and last, a
+# repeated pair
+ANTI = [([(0, 1), (1, 2), (2, 0), (2, 3), (3, 0)], [False, False, True, False, False]),
+        ([(0, 1), (1, 2), (2, 0)], [False, False, True]),
+        ([(0, 1), (1, 2), (2, 0), (1, 0), (2, 3)], [False, False, True, True, False]),
+        ([(0, 1), (0, 2), (1, 2), (2, 3), (0, 3)], [False, False, True, False, True]),
+        ([(0, 1), (1, 0), (0, 1)], [False, False, True]),
+        ([(0, 1), (1, 0)], [False, True])]
+
+
+def _rows_without_witness(g, edges, anti, d, EF, VF, gaps=None, epred=None, npred=None):
+    """Anti-edge semantics restated over sets: the oracle enumerates the motif
+    without its anti edges (roots below E - M + 1 of the whole motif); a row
+    counts when, for every anti edge j, no edge of the graph from m[a] to m[b]
+    lies after the position of the nearest earlier non-anti edge and within
+    edge j's window (gap from that edge, at most delta after the root)."""
+    M = len(edges)
+    keep = [j for j in range(M) if not anti[j]]
+    red = [edges[j] for j in keep]
+    _, prev = cc.anti_prev(anti, M)
+    Eg = np.asarray(g["Eg"])
+    E = g["E"]
+    if len(red) == 1:  # the root alone: every edge that is not a self-loop
+        rows = np.asarray(g["Euid"])[np.nonzero(Eg[:, 0] != Eg[:, 1])[0]].reshape(-1, 1)
+    else:
+        total, rows = oracle.enum(g, [(a, b, i) for i, (a, b) in enumerate(red)], d, 1 << 20)
+    epred, npred = epred or {}, npred or {}
+    out = 0
+    for rpos, m in cc.match_rows(g, rows.tolist(), red):
+        pos = [None] * M
+        for i, j in enumerate(keep):
+            pos[j] = rpos[i]
+        if pos[0] >= E - M + 1:
+            continue
+        t0 = int(Eg[pos[0], 2])
+        def tl(j):
+            tp = int(Eg[pos[prev[j]], 2])
+            if gaps is None:
+                return t0 + d
+            return tp + gaps[0] if j == 1 else min(tp + gaps[j - 1], t0 + d)
+        ok = all(int(Eg[pos[j], 2]) <= tl(j) for j in keep[1:])
+        ok = ok and all(epred[j](int(EF[pos[j]])) for j in keep if j in epred)
+        ok = ok and all(npred[i](int(VF[x])) for i, x in enumerate(m) if i in npred)
+        for j in range(1, M):
+            if not ok or not anti[j]:
+                continue
+            a, b = edges[j]
+            w = np.nonzero((Eg[:, 0] == m[a]) & (Eg[:, 1] == m[b]) & (np.arange(E) > pos[prev[j]])
+                           & (Eg[:, 2] <= tl(j)))[0]
+            if j in epred:
+                w = [q for q in w if epred[j](int(EF[q]))]
+            ok = len(w) == 0
+        out += ok
+    return out
+
+
+def test_anti_edges_are_rows_without_a_witness():
+    rng = np.random.default_rng(4)
+    nz = 0
+    for case in range(48):
+        U, V, T, g, EF, VF = _case(rng, int(rng.integers(20, 120)))
+        edges, anti = ANTI[case % len(ANTI)]
+        M = len(edges)
+        d = int(rng.integers(5, 80))
+        gaps = [int(rng.integers(0, d + 1)) for _ in range(M - 1)] if case % 3 == 1 else None
+        epred = {j: (lambda f: f % 3 != 0) for j in range(M)} if case % 4 == 2 else None
+        npred = {1: lambda f: f < 7} if case % 5 == 3 else None
+        want = _rows_without_witness(g, edges, anti, d, EF, VF, gaps, epred, npred)
+        got = cc.count_constrained(g, edges, d, EF, VF, node_pred=npred, edge_pred=epred, temporal=gaps, anti=anti)
+        assert got == want, (case, edges, anti, d, gaps)
+        nz += want > 0
+    assert nz > 20
+
+
+def test_anti_edges_partition_the_motif_without_them():
+    """Closed form for one anti edge last: the rows of the motif without it
+    split into those with a witness and those without, and those with a
+    witness at its first candidate are matches of the whole motif, so
+    count(without) - count(anti) counts distinct (row, first witness) pairs:
+    at most count(whole motif), equal to it when no row has two witnesses."""
+    rng = np.random.default_rng(5)
+    for case in range(16):
+        U, V, T, g, EF, VF = _case(rng, int(rng.integers(20, 100)))
+        edges = [(0, 1), (1, 2), (2, 0)]
+        d = int(rng.integers(5, 80))
+        E = g["E"]
+        red = oracle.count(g, [(0, 1, 0), (1, 2, 1)], d, end=E - 2, nthreads=1)
+        whole = oracle.count(g, [(a, b, i) for i, (a, b) in enumerate(edges)], d, nthreads=1)
+        got = cc.count_constrained(g, edges, d, EF, VF, anti=[False, False, True])
+        assert 0 <= red - got <= whole, (case, d, red, got, whole)
+        if whole == 0:
+            assert got == red

@@ -29,10 +29,10 @@ def test_option_defaults_and_overrides():
     assert q2.options().opts["warpCentric"] and q2.options().opts["tailExpansion"]
 
 
-def test_unsupported_features_and_multigpu_are_rejected():
+def test_anti_edge_query_accepted_and_multigpu_rejected():
     q = Query.Query(os.path.join(QDIR, "additional-features_WI-M4-all.yaml"))
-    with pytest.raises(NotImplementedError):
-        KernelGen.KernelGen(q.pattern(), q)
+    kg = KernelGen.KernelGen(q.pattern(), q)
+    assert kg.constrained and kg.anti == [False, False, True, False, False]
     q = Query.Query(os.path.join(QDIR, "multiGPU_ETH-M5-2.yaml"))
     with pytest.raises(NotImplementedError):
         KernelGen.KernelGen(q.pattern(), q)

@@ -1,5 +1,5 @@
 """Constraint features end to end on the GPU: a YAML query with label
-predicates on motif nodes / edges, temporal gaps and a filter goes through
+predicates on motif nodes / edges, temporal gaps, a filter and anti-edges goes through
 the frontend (MiningSingle: loader, generated constrained worker compiled
 for gfx950, SingleGPUExecution) and its count equals the constraint checker
 (tests/cons_check.py, pinned against the oracle) on the same graph and
@@ -37,6 +37,22 @@ CASES = [
                       'filter: "e1 < e2"'],
      {i: (lambda f: f % 3 == 1) for i in range(8)}, {0: lambda f: f % 2 == 1}, [900, 900],
      lambda el, nl: el[1] < el[2]),
+    # anti edges: the reference's M4-A example (predicates on the four nodes,
+    # temporal gaps), an anti edge last under an edge predicate, anti edges
+    # in the middle and last, two in a row with gaps, one with a filter
+    ("M4-A.txt", 8000, ["node0: f > 500000000", "node1: f < 500000000", "node2: f > 500000000",
+                        "node3: f < 500000000", "antiEdge: [False, False, True, False, False]",
+                        "temporal: [3000, 4000, 4000, 3000]"],
+     {0: lambda f: f > 500000000, 1: lambda f: f < 500000000, 2: lambda f: f > 500000000,
+      3: lambda f: f < 500000000}, {}, [3000, 4000, 4000, 3000], None, [False, False, True, False, False]),
+    ("cycle3.txt", 3000, ['edge: "f % 3 != 0"', "antiEdge: [false, false, true]"], {},
+     {j: (lambda f: f % 3 != 0) for j in range(8)}, None, None, [False, False, True]),
+    ("hub5.txt", 3000, ["antiEdge: [false, false, true, false, true]"], {}, {}, None, None,
+     [False, False, True, False, True]),
+    ("bitri6.txt", 6000, ["antiEdge: [false, false, false, true, true, false]", "temporal: [3000, 3000, 2500, 2500, 3000]"],
+     {}, {}, [3000, 3000, 2500, 2500, 3000], None, [False, False, False, True, True, False]),
+    ("M4-A.txt", 4000, ["antiEdge: [false, false, true, false, false]", 'filter: "(e0 + e4) % 2 == 0"'], {}, {},
+     None, lambda el, nl: (el[0] + el[4]) % 2 == 0, [False, False, True, False, False]),
 ]
 
 
@@ -69,7 +85,8 @@ def test_constraint_queries_through_the_frontend(world):
     root, g, EF, VF = world
     m = None
     nonzero = 0
-    for i, (pat, d, lines, npred, epred, tem, filt) in enumerate(CASES):
+    for i, (pat, d, lines, npred, epred, tem, filt, *anti) in enumerate(CASES):
+        anti = anti[0] if anti else None
         q = root / ("cq%d.yaml" % i)
         q.write_text("pattern: %s\ngraph: small.txt\nconstraints:\n  delta: %d\n%s\noptions:\n  gpu: 1\n" % (
             pat, d, "".join("  %s\n" % x for x in lines)))
@@ -80,8 +97,9 @@ def test_constraint_queries_through_the_frontend(world):
         else:
             cn, _, _ = frontend.Mining.run(m, str(q))
         edges = [(a, b) for a, b, _ in omotif.read_motif(os.path.join(ROOT, "inputs", "motifs", pat))]
-        want = cc.count_constrained(g, edges, d, EF, VF, node_pred=npred, edge_pred=epred, temporal=tem, filt=filt)
+        want = cc.count_constrained(g, edges, d, EF, VF, node_pred=npred, edge_pred=epred, temporal=tem, filt=filt,
+                                    anti=anti)
         say("  count", cn, "want", want)
         assert cn == want, (i, pat, lines, cn, want)
         nonzero += want > 0
-    assert nonzero >= 4
+    assert nonzero >= 8
