@@ -757,9 +757,21 @@ tm_status finish(tm_graph *g, Slot &sl, int q0, int nq, uint64_t *counts) {
 }
 
 // Counts of one motif over consecutive root ranges [cuts[i], cuts[i + 1])
-// (i < n <= TM_LAUNCH_MAXQ), in one launch of the batch kernel.
+// (i < n <= TM_LAUNCH_MAXQ), in one launch of the batch kernel; with a
+// caller's kernel (fn), one launch of it per range.
 static tm_status count_ranges(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta,
-                              const int32_t *cuts, int n, const tm_tuning *tune, uint64_t *counts) {
+                              const int32_t *cuts, int n, const tm_tuning *tune, uint64_t *counts,
+                              tm_launch_fn fn = nullptr, void *user = nullptr) {
+  if (fn) {
+    for (int i = 0; i < n; i++) {
+      int32_t b = cuts[i], e = cuts[i + 1];
+      tm_status st = stage_queries(g, sl, 0, 1, nullptr, &M, src, dst, M, &delta, &b, &e);
+      if (st == TM_OK) st = launch(g, sl, 0, 1, false, tune, false, false, fn, user);
+      if (st == TM_OK) st = finish(g, sl, 0, 1, counts + i);
+      if (st != TM_OK) return st;
+    }
+    return TM_OK;
+  }
   std::vector<int32_t> ne(n, M), dl(n, delta), bg(cuts, cuts + n), en(cuts + 1, cuts + n + 1);
   // query i: the same motif (stride 0), roots [cuts[i], cuts[i + 1])
   tm_status st = stage_queries(g, sl, 0, n, nullptr, ne.data(), src, dst, 0, dl.data(), bg.data(), en.data());
@@ -770,7 +782,8 @@ static tm_status count_ranges(tm_graph *g, Slot &sl, int M, const int32_t *src, 
 
 // Every row of the roots [b, e) (fewer than `room` of them) into host rows.
 static tm_status enum_all(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta,
-                          int32_t b, int32_t e, int64_t room, std::vector<int32_t> *rows) {
+                          int32_t b, int32_t e, int64_t room, std::vector<int32_t> *rows, tm_launch_fn fn,
+                          void *user) {
   int32_t *d_rows = nullptr;
   HIPTRY(hipMalloc(&d_rows, sizeof(int32_t) * (size_t)std::max<int64_t>(room, 1) * M));
   tm_status st = stage_queries(g, sl, 0, 1, nullptr, &M, src, dst, M, &delta, &b, &e);
@@ -778,7 +791,7 @@ static tm_status enum_all(tm_graph *g, Slot &sl, int M, const int32_t *src, cons
   if (st == TM_OK) {
     sl.s.h_queries[0].rows = d_rows;
     sl.s.h_queries[0].cap = room;
-    st = launch(g, sl, 0, 1, true, nullptr, false, false);
+    st = launch(g, sl, 0, 1, true, nullptr, false, false, fn, user);
   }
   if (st == TM_OK) st = finish(g, sl, 0, 1, &c);
   if (st == TM_OK && (int64_t)c >= room) st = fail(TM_ERR_INTERNAL, "enumeration found more rows than counted");
@@ -795,7 +808,7 @@ static tm_status enum_all(tm_graph *g, Slot &sl, int M, const int32_t *src, cons
 
 tm_status enum_cut(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta, int32_t beg,
                    int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows, uint64_t *count,
-                   int64_t *rows_written) {
+                   int64_t *rows_written, tm_launch_fn fn, void *user) {
   // the cut is found with the counting kernel's own launch parameters (its
   // class's), and the caller's choice of counting path
   *rows_written = 0;
@@ -823,7 +836,8 @@ tm_status enum_cut(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32
     const long long span = hi - lo;
     int n = (int)std::min<long long>(P, span);
     for (int i = 0; i <= n; i++) cuts[i] = lo + (int32_t)(span * i / n);
-    if ((st = count_ranges(g, sl, M, src, dst, delta, cuts.data(), n, ct, cnt.data())) != TM_OK) return st;
+    if ((st = count_ranges(g, sl, M, src, dst, delta, cuts.data(), n, fn ? tune : ct, cnt.data(), fn, user)) != TM_OK)
+      return st;
     int k = -1;
     uint64_t acc = before;
     for (int i = 0; i < n; i++) {
@@ -848,12 +862,13 @@ tm_status enum_cut(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32
   // rows of [beg, r*): all of them (fewer than cap); rows of r*: the
   // k = cap - before smallest by their Euid tuples
   std::vector<int32_t> head, tail;
-  if ((st = enum_all(g, sl, M, src, dst, delta, beg, rstar, (int64_t)before + 1, &head)) != TM_OK) return st;
+  if ((st = enum_all(g, sl, M, src, dst, delta, beg, rstar, (int64_t)before + 1, &head, fn, user)) != TM_OK) return st;
   if ((int64_t)head.size() != (int64_t)before * M) return fail(TM_ERR_INTERNAL, "enumeration and count disagree");
   const int64_t kMaxRootRows = (int64_t)1 << 27;
   if ((int64_t)c_star > kMaxRootRows)
     return fail(TM_ERR_NOMEM, "the root at the cap holds more than 2^27 matches; its rows cannot be ordered");
-  if ((st = enum_all(g, sl, M, src, dst, delta, rstar, rstar + 1, (int64_t)c_star + 1, &tail)) != TM_OK) return st;
+  if ((st = enum_all(g, sl, M, src, dst, delta, rstar, rstar + 1, (int64_t)c_star + 1, &tail, fn, user)) != TM_OK)
+    return st;
   if ((int64_t)tail.size() != (int64_t)c_star * M) return fail(TM_ERR_INTERNAL, "enumeration and count disagree");
   const int64_t k = cap - (int64_t)before;
   std::vector<int64_t> idx((size_t)c_star);
@@ -1399,24 +1414,35 @@ tm_status tm_batch_count(tm_graph *g, int32_t nq, const int32_t *num_edges, cons
 
 static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
                            int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows,
-                           uint64_t *count, int64_t *rows_written, tm_launch_fn fn, void *user);
+                           uint64_t *count, int64_t *rows_written, tm_launch_fn fn, void *user, bool cut_with_fn);
 
 tm_status tm_enum(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
                   int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows, uint64_t *count,
                   int64_t *rows_written) {
-  return enum_impl(g, num_edges, msrc, mdst, delta, beg, end, cap, tune, rows, count, rows_written, nullptr, nullptr);
+  return enum_impl(g, num_edges, msrc, mdst, delta, beg, end, cap, tune, rows, count, rows_written, nullptr, nullptr,
+                   false);
 }
 
 tm_status tm_enum_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
                          int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, tm_launch_fn launcher,
                          void *user, int32_t *rows, uint64_t *count, int64_t *rows_written) {
   if (!launcher) return fail(TM_ERR_ARG, "null launcher");
-  return enum_impl(g, num_edges, msrc, mdst, delta, beg, end, cap, tune, rows, count, rows_written, launcher, user);
+  return enum_impl(g, num_edges, msrc, mdst, delta, beg, end, cap, tune, rows, count, rows_written, launcher, user,
+                   false);
+}
+
+tm_status tm_enum_custom_counting(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                                  int32_t delta, int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune,
+                                  tm_launch_fn launcher, void *user, int32_t *rows, uint64_t *count,
+                                  int64_t *rows_written) {
+  if (!launcher) return fail(TM_ERR_ARG, "null launcher");
+  return enum_impl(g, num_edges, msrc, mdst, delta, beg, end, cap, tune, rows, count, rows_written, launcher, user,
+                   true);
 }
 
 static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst, int32_t delta,
                            int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows,
-                           uint64_t *count, int64_t *rows_written, tm_launch_fn fn, void *user) {
+                           uint64_t *count, int64_t *rows_written, tm_launch_fn fn, void *user, bool cut_with_fn) {
   if (!g || !msrc || !mdst || !count || cap < 0 || (cap > 0 && !rows)) return fail(TM_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(g->mtx);
   HIPTRY(hipSetDevice(g->device));
@@ -1444,7 +1470,10 @@ static tm_status enum_impl(tm_graph *g, int32_t num_edges, const int32_t *msrc, 
     (void)hipFree(d_rows);
     d_rows = nullptr;
     int64_t nw = 0;
-    st = enum_cut(g, sl, num_edges, msrc, mdst, delta, rbeg, rend, cap, tune, rows, count, &nw);
+    // the cut counts with the caller's kernel only when it counts (else the
+    // library's kernels, whose match semantics the caller's kernel shares)
+    st = enum_cut(g, sl, num_edges, msrc, mdst, delta, rbeg, rend, cap, tune, rows, count, &nw,
+                  cut_with_fn ? fn : nullptr, cut_with_fn ? user : nullptr);
     if (rows_written) *rows_written = st == TM_OK ? nw : 0;
   }
   if (d_rows) (void)hipFree(d_rows);

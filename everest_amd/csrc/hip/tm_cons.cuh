@@ -31,6 +31,13 @@
 // edge's label predicate applied) lies after pos[prev(j)] within its window;
 // the next edge's list and window start from pos[prev(j)] as well, and the
 // search backtracks to the nearest earlier non-anti edge.
+//
+// Enumeration (the query's rows pointer set): every complete match claims a
+// row index with one atomic and writes its edges' original ids (Euid) when
+// the index is below the cap, as the reference's enumOut
+// (system/tmine/KernelGen.py:509-525); the claim that reaches the cap sets the
+// query's stop flag and no root starts after it.  Counting: one atomic per
+// wavefront.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -38,6 +45,23 @@
 #include "tm_types.h"
 
 namespace tmc {
+
+// A complete match: counted, or its row claimed and written below the cap.
+template <int M>
+__device__ __forceinline__ void emit(const tm_query_dev &q, const tm_graph_view &g, const int (&pos)[M],
+                                     bool enumerate, unsigned long long &cnt) {
+  if (!enumerate) {
+    cnt++;
+    return;
+  }
+  const unsigned long long w = atomicAdd(q.count, 1ull);
+  if ((long long)w < q.cap) {
+    int *row = q.rows + (long long)w * M;
+#pragma unroll
+    for (int i = 0; i < M; i++) row[i] = g.Euid[pos[i]];
+  }
+  if ((long long)w + 1 >= q.cap) __hip_atomic_store(q.stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 template <class CS>
 __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
@@ -48,10 +72,13 @@ __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
   const long long delta = qt.x;
   const int beg = qt.y, end = qt.z;
   constexpr int M = CS::kM, N = CS::kN;
+  const tm_query_dev &q = args.queries[0];
+  const bool enumerate = q.rows != nullptr;
   unsigned long long cnt = 0;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long rr = beg + (long long)blockIdx.x * blockDim.x + threadIdx.x; rr < end; rr += stride) {
     const int r = (int)rr;
+    if (enumerate && __hip_atomic_load(q.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
     const tm_edge e0 = g.Eg[r];
     if (e0.u == e0.v) continue;  // roots with u == v are skipped (processFirstEdge.snp:1)
     if (!CS::edge_ok(0, EF, r) || !CS::node_ok(0, VF, e0.u) || !CS::node_ok(1, VF, e0.v)) continue;
@@ -102,7 +129,7 @@ __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
             open = true;
             continue;
           }
-          if (CS::filter(EF, VF, pos, m)) cnt++;
+          if (CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
         }
         j = pj;
         continue;
@@ -130,7 +157,7 @@ __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
       if (!CS::edge_ok(j, EF, p)) continue;
       pos[j] = p;
       if (j == M - 1) {
-        if (CS::filter(EF, VF, pos, m)) cnt++;
+        if (CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
         continue;
       }
       j++;
@@ -140,7 +167,7 @@ __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
   // wavefront sum, one atomic per wavefront; this launch's counter rows zeroed
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-  if (threadIdx.x == 0 && cnt) atomicAdd(args.queries[0].count, cnt);
+  if (threadIdx.x == 0 && cnt) atomicAdd(q.count, cnt);
   if (threadIdx.x < TM_NSTATS) args.wstats[(size_t)blockIdx.x * TM_NSTATS + threadIdx.x] = 0ull;
 }
 

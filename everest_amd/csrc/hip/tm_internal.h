@@ -127,9 +127,11 @@ const tm_tuning *tuning_for_class(tm_graph *g, int cls, const tm_tuning *tune);
 // M ints into host `rows`): every row of the roots before r*, the root at
 // which the cumulative count over [beg, r*] first reaches the cap, and the
 // remaining rows from root r*'s rows in increasing lexicographic order of
-// their Euid tuples.  *count = the matches of the roots [beg, r*].
+// their Euid tuples.  *count = the matches of the roots [beg, r*].  With a
+// caller's kernel (fn, user as tm_enum_custom's) every count and row comes
+// from that kernel.
 tm_status enum_cut(tm_graph *g, Slot &sl, int M, const int32_t *src, const int32_t *dst, int32_t delta, int32_t beg,
                    int32_t end, int64_t cap, const tm_tuning *tune, int32_t *rows, uint64_t *count,
-                   int64_t *rows_written);
+                   int64_t *rows_written, tm_launch_fn fn = nullptr, void *user = nullptr);
 
 }  // namespace tmh

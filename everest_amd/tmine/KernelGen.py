@@ -240,9 +240,11 @@ class KernelGen:
         self.cons = query.constraints()
         feats = self.cons.features()
         self.constrained = bool(feats)
-        if self.constrained and self.options.enumerate():
-            raise NotImplementedError("enumeration under constraint features is not supported by this build "
-                                      "(counting is)")
+        if "antiEdge" in feats and self.options.enumerate():
+            # the reference writes the id one past the graph's last edge for an anti edge's slot
+            raise NotImplementedError("enumeration with anti edges has no rows to match (an anti edge matches "
+                                      "no graph edge); counting with anti edges, and enumeration under the "
+                                      "other constraint features, are supported")
         if self.options.multigpu():
             raise NotImplementedError("this build drives one GPU (options.gpu = %d)" % self.options.gpu())
         self.pathToMotif = pathToMotif
@@ -276,7 +278,7 @@ class KernelGen:
         if self.constrained:
             subs = dict(motif_name=os.path.basename(self.pathToMotif), M=tb["M"], N=tb["N"],
                         options=", ".join("%s=%s" % kv for kv in sorted(self.options.opts.items())),
-                        cons_text=" ".join(str(self.cons.cons).split()),
+                        cons_text=" ".join(str(self.cons.cons).split()), enum_cap=self.enum_NUM,
                         src=join(a for a, _ in self.edges), dst=join(b for _, b in self.edges),
                         kind=join(tb["kind"]), a=join(tb["a"]), b=join(tb["b"]), mapped=join(tb["mapped"]))
             subs.update(constraint_bodies(self.cons, tb["M"], tb["N"], self.anti))

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define TM_ABI_VERSION 6
+#define TM_ABI_VERSION 7
 #define TM_MAX_MOTIF_EDGES 8
 #define TM_MAX_MOTIF_NODES 8
 
@@ -285,6 +285,21 @@ tm_status tm_enum_custom(tm_graph *g, int32_t num_edges, const int32_t *msrc, co
                          int32_t delta, int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune,
                          tm_launch_fn launcher, void *user, int32_t *rows, uint64_t *count,
                          int64_t *rows_written);
+
+/*
+ * tm_enum_custom for a kernel that also counts: launched with the staged
+ * query's rows pointer null, it counts the query's matches.  At the cap the
+ * deterministic cut (tm_enum) is then found with the plugin's kernel too
+ * (tm_enum_custom finds it with the library's own kernels, which is right
+ * only for a kernel with the library's match semantics).  For a generated
+ * worker under constraint features (label predicates, temporal gaps, a
+ * filter: system/tmine/KernelGen.py:467-534), whose matches only its own
+ * kernel knows.
+ */
+tm_status tm_enum_custom_counting(tm_graph *g, int32_t num_edges, const int32_t *msrc, const int32_t *mdst,
+                                  int32_t delta, int32_t beg, int32_t end, int64_t cap, const tm_tuning *tune,
+                                  tm_launch_fn launcher, void *user, int32_t *rows, uint64_t *count,
+                                  int64_t *rows_written);
 
 /*
  * Shape classes and the graph's tuning table (the autotuner's output).  A

@@ -12,7 +12,8 @@ node_pred / edge_pred: {motif node or edge index: callable(label) -> bool};
 temporal: gaps [g0, g1, ...] (edge j + 1 at most g_j after edge j, edge 1 at
 most g0 after the root, every edge at most delta after the root from edge 2
 on); filt: callable(edge_labels, node_labels) -> bool over a complete match;
-EF: label per edge position, VF: label per (renamed) vertex.
+EF: label per edge position, VF: label per (renamed) vertex; out_rows, when
+given, collects the matches' edge positions (without anti edges).
 """
 import numpy as np
 
@@ -44,7 +45,7 @@ def anti_prev(anti, M):
 
 
 def count_constrained(g, edges, delta, EF, VF, node_pred=None, edge_pred=None, temporal=None, filt=None,
-                      beg=0, end=None, anti=None):
+                      beg=0, end=None, anti=None, out_rows=None):
     node_pred, edge_pred = node_pred or {}, edge_pred or {}
     Eg, E = g["Eg"], g["E"]
     T = Eg[:, 2]
@@ -110,6 +111,8 @@ def count_constrained(g, edges, delta, EF, VF, node_pred=None, edge_pred=None, t
             if j == M - 1:
                 if filt is None or filt([int(EF[x]) for x in pos + [q]], [int(VF[x]) for x in mm]):
                     total += 1
+                    if out_rows is not None:
+                        out_rows.append(tuple(pos + [q]))
             else:
                 dfs(j + 1, mm, pos + [q], t0)
 

@@ -1,8 +1,10 @@
 """The constraint checker (tests/cons_check.py) against the oracle: no
 constraints gives the oracle's count; a uniform node or edge predicate gives
 the oracle's count on the edge list restricted to the edges it keeps;
-per-node and per-edge predicates, temporal gaps and filters give the number
-of the oracle's enumerated rows that satisfy them."""
+per-node and per-edge predicates, temporal gaps and filters give the
+oracle's enumerated rows that satisfy them (their number and the rows
+themselves); anti-edges give the oracle's rows of the motif without them
+that have no witness edge."""
 import numpy as np
 
 from oracle import oracle, prep
@@ -73,6 +75,7 @@ def test_per_item_predicates_temporal_and_filter_are_filtered_rows():
         filt = lambda el, nl: (el[0] + nl[0]) % 2 == 0
         total, rows = oracle.enum(g, mot, d, 1 << 20)
         ok = 0
+        want_rows = []
         for pos, m in cc.match_rows(g, rows.tolist(), edges):
             t = [int(g["Eg"][p, 2]) for p in pos]
             if any(t[j + 1] > t[j] + gaps[j] for j in range(M - 1)):
@@ -82,8 +85,12 @@ def test_per_item_predicates_temporal_and_filter_are_filtered_rows():
             if not filt([int(EF[p]) for p in pos], [int(VF[x]) for x in m]):
                 continue
             ok += 1
-        got = cc.count_constrained(g, edges, d, EF, VF, node_pred=npred, edge_pred=epred, temporal=gaps, filt=filt)
-        assert got == ok, (case, edges, d, gaps)
+            want_rows.append(tuple(pos))
+        got_rows = []
+        got = cc.count_constrained(g, edges, d, EF, VF, node_pred=npred, edge_pred=epred, temporal=gaps, filt=filt,
+                                   out_rows=got_rows)
+        assert got == ok == len(got_rows), (case, edges, d, gaps)
+        assert sorted(got_rows) == sorted(want_rows), (case, edges, d, gaps)
         nz += ok > 0
     assert nz > 10
 

@@ -103,3 +103,45 @@ def test_constraint_queries_through_the_frontend(world):
         assert cn == want, (i, pat, lines, cn, want)
         nonzero += want > 0
     assert nonzero >= 8
+
+
+# enumerating constraint queries: (motif, delta, YAML lines, node preds, edge preds, temporal, filter)
+ENUM_CASES = [
+    ("cycle3.txt", 3000, ['node: "f % 2 == 0"', "temporal: [2000, 1500]", 'filter: "e0 < e2"'],
+     {i: (lambda f: f % 2 == 0) for i in range(8)}, {}, [2000, 1500], lambda el, nl: el[0] < el[2]),
+    ("M4.txt", 4000, ['edge: "f % 3 != 0"', 'node2: "f < 700000000"'], {2: lambda f: f < 700000000},
+     {j: (lambda f: f % 3 != 0) for j in range(8)}, None, None),
+]
+
+
+def test_constraint_enumeration_through_the_frontend(world):
+    """Rows of enumerating constraint queries: below the cap exactly the
+    checker's rows (original edge ids); at a cap, the deterministic cut (the
+    rows of the roots before the root that reaches the cap, then that root's
+    smallest rows) computed from the checker's rows in root order."""
+    from everest_amd.tmine import frontend
+    from helpers import deterministic_cut
+    root, g, EF, VF = world
+    Euid = np.asarray(g["Euid"])
+    m = None
+    for i, (pat, d, lines, npred, epred, tem, filt) in enumerate(ENUM_CASES):
+        edges = [(a, b) for a, b, _ in omotif.read_motif(os.path.join(ROOT, "inputs", "motifs", pat))]
+        pos_rows = []
+        total = cc.count_constrained(g, edges, d, EF, VF, node_pred=npred, edge_pred=epred, temporal=tem, filt=filt,
+                                     out_rows=pos_rows)
+        assert total == len(pos_rows) and total > 30
+        all_rows = Euid[np.asarray(pos_rows, dtype=np.int64)]
+        for cap in (total + 10, total // 3):
+            q = root / ("ce%d_%d.yaml" % (i, cap))
+            q.write_text("pattern: %s\ngraph: small.txt\nconstraints:\n  delta: %d\n%s\noptions:\n  gpu: 1\n"
+                         "  enumerate: %d\n" % (pat, d, "".join("  %s\n" % x for x in lines), cap))
+            say("constraint enumeration", i, pat, d, lines, "cap", cap, "total", total)
+            if m is None:
+                m = frontend.MiningSingle(str(q))
+                cn, _, _, rows = m.run()
+            else:
+                cn, _, _, rows = frontend.Mining.run(m, str(q))
+            want_rows, want_cnt = deterministic_cut(g, all_rows, cap)
+            say("  count", cn, "want", want_cnt, "rows", len(rows))
+            assert cn == want_cnt
+            assert sorted(map(tuple, rows.tolist())) == sorted(want_rows)

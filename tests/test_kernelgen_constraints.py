@@ -1,9 +1,9 @@
 """KernelGen for constraint features (no GPU): the generated constrained
 worker holds the predicates, gaps and filter as device code and compiles
 with hipcc for gfx950, anti-edges included (the reference's M4-A example);
-anti-edges with a new endpoint or directly after the root, filters naming
-an anti edge, and enumeration under constraints are rejected; malformed
-expressions are refused."""
+enumeration included; anti-edges with a new endpoint or directly after the
+root, filters naming an anti edge, and enumeration with anti-edges are
+rejected; malformed expressions are refused."""
 import os
 import tempfile
 
@@ -25,8 +25,9 @@ def test_constrained_worker_generates_and_compiles(monkeypatch):
     monkeypatch.setenv("EVEREST_INPUTS", os.path.join(ROOT, "inputs"))
     with tempfile.TemporaryDirectory() as tmp:
         q = Query.Query(_query(tmp, ['node: "f % 2 == 0"', 'edge3: "f > 7"', "temporal: [5, 6, 7, 8]",
-                                     'filter: "e0 < e4 && n2 != n3"']))
+                                     'filter: "e0 < e4 && n2 != n3"'], "  enumerate: 1000\n"))
         code = KernelGen.KernelGen(q.pattern(), q).generate()
+        assert "constexpr long long kEnumCap = 1000;" in code
         assert "tmc::cons_kernel<CSpec>" in code
         assert "case 3: return (f % 2 == 0);" in code and "case 3: return (f > 7);" in code
         assert "{5, 6, 7, 8, 0, 0, 0, 0}" in code
@@ -55,7 +56,8 @@ def test_unsupported_constraint_forms(monkeypatch):
         q = Query.Query(_query(tmp, ["antiEdge: [false, false, true, false, false]", 'filter: "e2 > 1"']))
         with pytest.raises(ValueError):
             KernelGen.KernelGen(q.pattern(), q).generate()
-        q = Query.Query(_query(tmp, ['node: "f > 1"'], "  enumerate: 10\n"))
+        q = Query.Query(_query(tmp, ["antiEdge: [false, false, true, false, false]"], "  enumerate: 10\n",
+                               pattern="M4-A.txt"))
         with pytest.raises(NotImplementedError):
             KernelGen.KernelGen(q.pattern(), q)
         for bad in ['node: "f > 1; return 0"', 'filter: "e9 > 1"', "temporal: [1, 2]"]:
