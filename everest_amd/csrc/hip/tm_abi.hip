@@ -383,6 +383,22 @@ static bool sweep_pays(const tm_graph *g, const SweepItem &it, int delta) {
   return d2 * frac >= 8.0 * (double)g->E;
 }
 
+// The root-pair sweep replaces the expansion of every root's edge-1 slots up
+// to its pair's last edge in its window by one expansion per pair: it pays
+// when those windows are long (the same estimate as the window sweep's, over
+// edge 1's list, against PAIR_SLOTS mean slots per root).  Measured on a
+// 64M-edge power-law graph: a fixed 18 ms; 0 -> 1, 0 -> 2, 3 -> 2, 0 -> 1 at
+// estimates 2 / 12 / 50 / 80 / 274: sweep 20 / 37 / 162 / 287 / 1293 ms,
+// mining 4 / 36 / 289 / 614 / 4624 ms; 0 -> 2, 2 -> 3 at 50 / 346: sweep
+// 34 / 150, mining 22 / 253.
+static const double PAIR_SLOTS = 40.0;
+static bool pair_pays(const tm_graph *g, const PairItem &it, int delta) {
+  const bool root_out = it.s1 == 0, win_out = it.d1 != 0;
+  const double d2 = root_out == win_out ? g->deg2[root_out ? 0 : 1] : g->deg2[2];
+  const double frac = g->tspan > 0 ? std::min(1.0, (double)delta / g->tspan) : 1.0;
+  return d2 * frac >= PAIR_SLOTS * (double)g->E;
+}
+
 // Kernel variants (tm_variants.h): node map of 4 or 8, count / enumerate /
 // instrumented count; counting also comes with register caps of 96 and 80
 // VGPRs, and in a single-query and a multi-query (batch) form.
@@ -405,14 +421,21 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   // caller's tuning keeps them on the mining kernel); the mining launch
   // sees them with an empty root range
   pl->sweep.clear();
+  pl->pair.clear();
   std::vector<char> swept(nq, 0);
   if (!enumerate && instr == 0 && !fn && !(tune && tune->sweep < 0)) {
     for (int i = 0; i < nq; i++) {
       SweepItem it;
+      PairItem pi;
       if (sweep_shape(s.h_queries[q0 + i].m, &it) &&
           ((tune && tune->sweep > 0) || sweep_pays(g, it, s.h_queries[q0 + i].delta))) {
         it.q = i;
         pl->sweep.push_back(it);
+        swept[i] = 1;
+      } else if (pair_shape(s.h_queries[q0 + i].m, &pi) &&
+                 ((tune && tune->sweep > 0) || pair_pays(g, pi, s.h_queries[q0 + i].delta))) {
+        pi.q = i;
+        pl->pair.push_back(pi);
         swept[i] = 1;
       }
     }
@@ -574,7 +597,7 @@ tm_status plan(tm_graph *g, Slot &sl, int q0, int nq, bool enumerate, const tm_t
   sl.last_grid = grid;
   sl.last_lds = lds;
   sl.last_wpe = fn ? 0 : wpe;
-  sl.last_sweep = (int)pl->sweep.size();
+  sl.last_sweep = (int)(pl->sweep.size() + pl->pair.size());
   pl->a = a;
   pl->d_counts = s.d_counts + q0;
   pl->d_stops = s.d_stops + q0;
@@ -632,6 +655,12 @@ tm_status enqueue(tm_graph *g, Slot &sl, const Plan &pl, bool timing, bool count
     b.n = (int)std::min<size_t>(TM_SWEEP_MAXQ, pl.sweep.size() - i);
     for (int k = 0; k < b.n; k++) b.item[k] = pl.sweep[i + k];
     if ((st = sweep_enqueue(g->view, pl.a.queries, b, sl.stream)) != TM_OK) return st;
+  }
+  for (size_t i = 0; i < pl.pair.size(); i += TM_SWEEP_MAXQ) {
+    PairBatch b;
+    b.n = (int)std::min<size_t>(TM_SWEEP_MAXQ, pl.pair.size() - i);
+    for (int k = 0; k < b.n; k++) b.item[k] = pl.pair[i + k];
+    if ((st = pair_enqueue(g->view, pl.a.queries, b, sl.stream)) != TM_OK) return st;
   }
   if (timing) HIPTRY(hipEventRecord(sl.ev1, sl.stream));
   hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, sl.stream, pl.a.wstats, pl.grid, pl.a.sched);

@@ -98,6 +98,7 @@ struct Plan {
   tm_launch_fn fn = nullptr;
   void *user = nullptr;
   std::vector<SweepItem> sweep;  // queries counted by the window-sweep kernel (empty root range in the mining launch)
+  std::vector<PairItem> pair;    // queries counted by the root-pair sweep kernel (likewise)
 };
 // Plan a launch of staged queries [q0, q0 + nq) of the slot's staging
 // (occupancy, variant, claim size, query table) and upload its tables.

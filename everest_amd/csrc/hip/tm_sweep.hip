@@ -117,9 +117,173 @@ __global__ __launch_bounds__(256) void sweep_kernel(tm_graph_view g, const tm_qu
   if ((threadIdx.x & 63) == 0 && acc) atomicAdd(qd.count, acc);
 }
 
+// Root-pair sweep: 4-edge motifs root 0 -> 1, edge 1 a new node 2 on a list
+// of node s1 in {0, 1} (L1: out-list when d1, else in-list), edge 2 a new
+// node 3 on a list of node 2 (L2: out when d2, else in), edge 3 the root's
+// pair 0 -> 1 again.  Each root r = (u, v) of the mining kernel expands the
+// L1 slots in (r, clip_r), clip_r its pair's last edge in its window, and
+// the roots of one pair overlap there.  Here the L1 slots of a pair are
+// expanded once: root i takes the piece of (p_i, clip_i) past the previous
+// root's clip, and an L1 slot a (new node w) counts, per pair edge p_j > a,
+// the roots before a whose window reaches p_j times the edge-2 slots between:
+//   sum_j (A - lo_j) * n2(a, p_j)  over the j with lo_j < A,
+// A = the pair's roots (within [beg, end)) before a, lo_j = its first root
+// with time >= t_j - delta (nondecreasing in j, so the sum stops at the first
+// j with lo_j >= A), n2 = L2(w) slots in (a, p_j) whose other end is not u,
+// v or w.  A wavefront takes 64 consecutive roots, one per lane, and deals
+// the union of their pieces over its lanes (a scan of the piece lengths,
+// each slot's root found by a binary search over the lanes), so a hub's
+// long piece is shared by all 64 lanes.  Runs to completion, like the
+// window sweep.
+__global__ __launch_bounds__(256) void pair_sweep_kernel(tm_graph_view g, const tm_query_dev *queries,
+                                                         tmh::PairBatch b) {
+  const tmh::PairItem it = b.item[blockIdx.y];
+  const tm_query_dev &qd = queries[it.q];
+  const int beg = qd.root_beg, end = qd.root_end, delta = qd.delta;
+  const int lane = threadIdx.x & 63;
+  const long long groups = ((long long)end - beg + 63) / 64;
+  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
+  const int *L1R = it.d1 ? g.outR : g.inR;
+  const int *L1P = it.d1 ? g.outPos : g.inPos;
+  const int *L1N = it.d1 ? g.outNbr : g.inNbr;
+  const int *L2R = it.d2 ? g.outR : g.inR;
+  const int *L2P = it.d2 ? g.outPos : g.inPos;
+  const int *L2N = it.d2 ? g.outNbr : g.inNbr;
+  const int *pp = g.outPairPos, *pn = g.outPairNbr;
+  constexpr int kScan = 32;
+  unsigned long long acc = 0;
+  for (long long gi = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); gi < groups; gi += wstride) {
+    // this lane's root: its pair run [s0, s1e), its roots from slot rb, its L1 piece [i0, i0 + n)
+    const long long rr = beg + gi * 64 + lane;
+    int n = 0, u = 0, v = 0, s0 = 0, s1e = 0, rb = 0, i0 = 0;
+    if (rr < end) {
+      const int r = (int)rr;
+      const tm_edge e = g.Eg[r];
+      u = e.u;
+      v = e.v;
+      if (u != v) {
+        const int k = g.outPairSelf[r];
+        s0 = lower_bound_i(pn, g.outR[u], k, v);
+        s1e = upper_bound_i(pn, k + 1, g.outR[u + 1], v);
+        const int clip = pp[lower_bound_i(pp, k, s1e, pend_of(g, (long long)e.t + delta)) - 1];
+        int lo = r;
+        if (k > s0 && pp[k - 1] >= beg) {
+          const int pr = pp[k - 1];
+          lo = max(lo, pp[lower_bound_i(pp, k - 1, s1e, pend_of(g, (long long)g.T[pr] + delta)) - 1]);
+        }
+        if (clip > lo) {
+          const int x = it.s1 == 0 ? u : v;
+          i0 = upper_bound_i(L1P, L1R[x], L1R[x + 1], lo);
+          n = lower_bound_i(L1P, i0, L1R[x + 1], clip) - i0;
+          rb = lower_bound_i(pp, s0, s1e, beg);
+        }
+      }
+    }
+    long long incl = n;  // 64 pieces of up to E slots each: 64-bit
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    const long long total = __shfl(incl, 63, 64);
+    for (long long base = 0; base < total; base += 64) {
+      const long long item = base + lane;
+      // owner: the first lane whose inclusive sum exceeds the item
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (__shfl(incl, mid, 64) > item) hi = mid;
+        else lo = mid + 1;
+      }
+      const int own = lo;
+      const long long oincl = __shfl(incl, own, 64);
+      const int on = __shfl(n, own, 64), oi0 = __shfl(i0, own, 64);
+      const int ou = __shfl(u, own, 64), ov = __shfl(v, own, 64);
+      const int os0 = __shfl(s0, own, 64), os1 = __shfl(s1e, own, 64), orb = __shfl(rb, own, 64);
+      if (item >= total) continue;
+      const int slot = oi0 + (int)(item - (oincl - on));
+      const int a = L1P[slot], w = L1N[slot];
+      if (w == ou || w == ov) continue;
+      const int A = lower_bound_i(pp, orb, os1, min(a, end));
+      const int l2lo = upper_bound_i(L2P, L2R[w], L2R[w + 1], a), l2hi = L2R[w + 1];
+      // edge-2 slots of w after a are scanned once for all j (their other
+      // ends checked directly) while the scan stays short; past kScan slots
+      // each j's count is a rank minus the pair counts of u, v and w
+      int cur = l2lo;
+      long long valid = 0;
+      bool scan = true;
+      int loj = -1;
+      for (int j = upper_bound_i(pp, os0, os1, a); j < os1; j++) {
+        const int pj = pp[j];
+        const long long tj = (long long)g.T[pj] - delta;  // roots from the first with time >= tj
+        if (loj < 0) {
+          int lo2 = orb, hi2 = A;
+          while (lo2 < hi2) {
+            const int mid = (lo2 + hi2) >> 1;
+            if ((long long)g.T[pp[mid]] < tj) lo2 = mid + 1;
+            else hi2 = mid;
+          }
+          loj = lo2;
+        } else {
+          while (loj < A && (long long)g.T[pp[loj]] < tj) loj++;
+        }
+        if (loj >= A) break;
+        long long n2;
+        if (scan) {
+          while (cur < l2hi && cur - l2lo < kScan && L2P[cur] < pj) {
+            const int y = L2N[cur++];
+            valid += (y != ou) & (y != ov) & (y != w);
+          }
+          scan = !(cur < l2hi && L2P[cur] < pj);  // every slot before pj scanned
+        }
+        if (scan) {
+          n2 = valid;
+        } else {
+          n2 = lower_bound_i(L2P, l2lo, l2hi, pj) - l2lo;
+          if (it.d2) n2 -= edge_count(g, w, ou, a, pj) + edge_count(g, w, ov, a, pj);
+          else n2 -= edge_count(g, ou, w, a, pj) + edge_count(g, ov, w, a, pj);
+          if (!g.no_self_loops) n2 -= edge_count(g, w, w, a, pj);
+        }
+        acc += (unsigned long long)(A - loj) * (unsigned long long)n2;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0 && acc) atomicAdd(qd.count, acc);
+}
+
 }  // namespace
 
 namespace tmh {
+
+int pair_shape(const tm_motif_dev &md, PairItem *it) {
+  if (md.M != 4 || md.kind[3] != TM_K_CLOSE || md.a[3] != 0 || md.b[3] != 1) return 0;
+  int s1, d1, d2;
+  if (md.kind[1] == TM_K_NEW_DST && md.a[1] <= 1) s1 = md.a[1], d1 = 1;
+  else if (md.kind[1] == TM_K_NEW_SRC && md.b[1] <= 1) s1 = md.b[1], d1 = 0;
+  else return 0;
+  if (md.kind[2] == TM_K_NEW_DST && md.a[2] == 2) d2 = 1;
+  else if (md.kind[2] == TM_K_NEW_SRC && md.b[2] == 2) d2 = 0;
+  else return 0;
+  if (it) {
+    it->s1 = s1;
+    it->d1 = d1;
+    it->d2 = d2;
+  }
+  return 1;
+}
+
+tm_status pair_enqueue(const tm_graph_view &g, const tm_query_dev *d_queries, const PairBatch &b, hipStream_t stream) {
+  if (b.n <= 0) return TM_OK;
+  // one wavefront per 64 roots, at most 4096 blocks of 4 wavefronts per query
+  const long long per = ((long long)g.E + 255) / 256;
+  const unsigned gx = (unsigned)std::max<long long>(1, std::min<long long>(per, 4096));
+  hipLaunchKernelGGL(pair_sweep_kernel, dim3(gx, (unsigned)b.n), dim3(256), 0, stream, g, d_queries, b);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(TM_ERR_HIP, std::string("pair-sweep launch: ") + hipGetErrorString(e));
+  return TM_OK;
+}
 
 int sweep_shape(const tm_motif_dev &md, SweepItem *it) {
   if (md.M != 3) return 0;

@@ -92,7 +92,11 @@ typedef struct tm_tuning {
                             root node (a star of two new nodes, or a new node's pair twice): > 0 = counted by
                             the window-sweep kernel (O(E log E) per query whatever delta), < 0 = by the
                             mining kernel like every other motif, 0 = the sweep when the roots' windows
-                            are estimated (node degrees x delta / time span) at 8 or more slots on average */
+                            are estimated (node degrees x delta / time span) at 8 or more slots on average.
+                            The same choice for 4-edge motifs whose last edge repeats the root's pair
+                            (edge 1 a new node on a root node's list, edge 2 a new node on edge 1's new
+                            node's list): counted by the root-pair sweep kernel, which expands each
+                            pair's edge-1 slots once for all of the pair's roots */
 } tm_tuning;
 
 /* Library version (TM_ABI_VERSION) and last error of the calling thread. */
@@ -186,7 +190,7 @@ tm_status tm_device_free(tm_graph *g, void *device_ptr);
  * work, summed, and the launch shape: [12]
  * wavefronts in the grid, [13] blocks per claim, [14] LDS bytes per
  * wavefront, [15] register-cap variant, [16] queries of the launch counted
- * by the window-sweep kernel (their work is not in [0..11]).  Up to 17
+ * by the window-sweep or root-pair sweep kernels (their work is not in [0..11]).  Up to 17
  * entries.
  * No reference counterpart (instrumentation).
  */

@@ -1392,3 +1392,96 @@ def window_sweep_count(g, md, delta, beg, end):
             s = valid(ia) * add - (valid(ir) * rem if ir < r1 else 0)
             tot += s if pat == WS_PAIR else -s
     return tot % (1 << 64)
+
+
+# Root-pair sweep (the pair-sweep kernel): 4-edge motifs root 0 -> 1, edge 1
+# a new node 2 on a list of node 0 or 1, edge 2 a new node 3 on a list of
+# node 2, edge 3 the root's pair 0 -> 1 again.  The roots of one pair (u, v)
+# share their expansions: an edge-1 slot a is expanded once, by the root
+# whose window piece holds it (pieces: root i's window (p_i, clip_i) past the
+# previous root's clip), and weighted per closing pair edge p_j > a by the
+# roots before a whose window reaches p_j:
+#   sum_j (A - lo_j) * #(valid edge-2 slots in (a, p_j)),  lo_j < A,
+# A = roots of the pair before a, lo_j = the first root with time >= t_j - delta.
+
+def pair_sweep_plan(md):
+    """(s1, d1, d2): edge 1 on node s1's out-list (d1 = 1) or in-list, edge 2
+    on node 2's out-list (d2 = 1) or in-list; None for other motifs."""
+    if md["M"] != 4 or md["kind"][3] != K_CLOSE or (md["a"][3], md["b"][3]) != (0, 1):
+        return None
+    k1, a1, b1, k2, a2, b2 = md["kind"][1], md["a"][1], md["b"][1], md["kind"][2], md["a"][2], md["b"][2]
+    if k1 == K_NEW_DST and a1 in (0, 1):
+        s1, d1 = a1, 1
+    elif k1 == K_NEW_SRC and b1 in (0, 1):
+        s1, d1 = b1, 0
+    else:
+        return None
+    if k2 == K_NEW_DST and a2 == 2:
+        d2 = 1
+    elif k2 == K_NEW_SRC and b2 == 2:
+        d2 = 0
+    else:
+        return None
+    return (s1, d1, d2)
+
+
+def pair_sweep_count(g, md, delta, beg, end, scan_max=32):
+    """The pair-sweep kernel's count (mod 2^64), root by root; edge-2 slots
+    scanned directly while at most scan_max of them precede p_j, else
+    counted by rank minus pair counts (the kernel's kScan)."""
+    s1, d1, d2 = pair_sweep_plan(md)
+    Eg, T = g["Eg"], g["T"]
+    pp, pn, oR = g["outPairPos"], g["outPairNbr"], g["outR"]
+    L1R, L1P, L1N = (g["outR"], g["outPos"], g["outNbr"]) if d1 else (g["inR"], g["inPos"], g["inNbr"])
+    L2R, L2P, L2N = (g["outR"], g["outPos"], g["outNbr"]) if d2 else (g["inR"], g["inPos"], g["inNbr"])
+    tot = 0
+    for r in range(beg, end):
+        u, v, t = (int(x) for x in Eg[r])
+        if u == v:
+            continue
+        k = int(g["outPairSelf"][r])
+        s0 = lower_bound(pn, oR[u], k, v)
+        s1_ = upper_bound(pn, k, oR[u + 1], v)
+        c = lower_bound(pp, k, s1_, pend_of(g, t + delta)) - 1
+        clip = int(pp[c])
+        lo = r
+        if k > s0 and pp[k - 1] >= beg:  # the previous root's window covers up to its clip
+            cp = lower_bound(pp, k - 1, s1_, pend_of(g, int(T[pp[k - 1]]) + delta)) - 1
+            lo = max(lo, int(pp[cp]))
+        if clip <= lo:
+            continue
+        x = u if s1 == 0 else v
+        i0 = upper_bound(L1P, L1R[x], L1R[x + 1], lo)
+        i1 = lower_bound(L1P, i0, L1R[x + 1], clip)
+        rb = lower_bound(pp, s0, s1_, beg)
+        for sl in range(i0, i1):
+            a, w = int(L1P[sl]), int(L1N[sl])
+            if w == u or w == v:
+                continue
+            A = lower_bound(pp, rb, s1_, min(a, end))
+            j = upper_bound(pp, s0, s1_, a)
+            base = upper_bound(L2P, L2R[w], L2R[w + 1], a)
+            hi2 = L2R[w + 1]
+            cur, valid, scan = base, 0, True
+            while j < s1_:
+                pj = int(pp[j])
+                lo_j = max(rb, lower_bound(pp, s0, s1_, pend_of(g, int(T[pj]) - delta - 1)))
+                if lo_j >= A:
+                    break
+                if scan:
+                    while cur < hi2 and cur - base < scan_max and L2P[cur] < pj:
+                        y = int(L2N[cur])
+                        valid += (y != u) and (y != v) and (y != w)
+                        cur += 1
+                    scan = not (cur < hi2 and L2P[cur] < pj)
+                if scan:
+                    n2 = valid
+                else:
+                    n2 = lower_bound(L2P, base, hi2, pj) - base
+                    if d2:
+                        n2 -= edge_count(g, w, u, a, pj) + edge_count(g, w, v, a, pj) + edge_count(g, w, w, a, pj)
+                    else:
+                        n2 -= edge_count(g, u, w, a, pj) + edge_count(g, v, w, a, pj) + edge_count(g, w, w, a, pj)
+                tot += (A - lo_j) * n2
+                j += 1
+    return tot % (1 << 64)
