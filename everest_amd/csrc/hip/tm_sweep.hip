@@ -204,8 +204,14 @@ __global__ __launch_bounds__(256) void pair_sweep_kernel(tm_graph_view g, const 
       const int slot = oi0 + (int)(item - (oincl - on));
       const int a = L1P[slot], w = L1N[slot];
       if (w == ou || w == ov) continue;
+      // w's edge-2 list after a, from the upload's rank arrays (a lies in
+      // in(w) when edge 1 enters w, in out(w) when it leaves w)
+      const int l2lo = it.d1 ? (it.d2 ? g.outNext[a] : g.inSelf[a] + 1) : (it.d2 ? g.outSelf[a] + 1 : g.inNext[a]);
+      const int l2hi = L2R[w + 1];
+      if (l2lo >= l2hi) continue;
+      const int first2 = L2P[l2lo];  // no j with p_j <= first2 adds anything
       const int A = lower_bound_i(pp, orb, os1, min(a, end));
-      const int l2lo = upper_bound_i(L2P, L2R[w], L2R[w + 1], a), l2hi = L2R[w + 1];
+      const long long tlim = (long long)g.T[pp[A - 1]] + delta;  // p_j beyond it: no root before a reaches it
       // edge-2 slots of w after a are scanned once for all j (their other
       // ends checked directly) while the scan stays short; past kScan slots
       // each j's count is a rank minus the pair counts of u, v and w
@@ -215,7 +221,10 @@ __global__ __launch_bounds__(256) void pair_sweep_kernel(tm_graph_view g, const 
       int loj = -1;
       for (int j = upper_bound_i(pp, os0, os1, a); j < os1; j++) {
         const int pj = pp[j];
-        const long long tj = (long long)g.T[pj] - delta;  // roots from the first with time >= tj
+        const long long tpj = g.T[pj];
+        if (tpj > tlim) break;
+        if (pj <= first2) continue;
+        const long long tj = tpj - delta;  // roots from the first with time >= tj
         if (loj < 0) {
           int lo2 = orb, hi2 = A;
           while (lo2 < hi2) {

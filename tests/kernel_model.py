@@ -1458,16 +1458,29 @@ def pair_sweep_count(g, md, delta, beg, end, scan_max=32):
             a, w = int(L1P[sl]), int(L1N[sl])
             if w == u or w == v:
                 continue
-            A = lower_bound(pp, rb, s1_, min(a, end))
-            j = upper_bound(pp, s0, s1_, a)
-            base = upper_bound(L2P, L2R[w], L2R[w + 1], a)
+            # edge-2 list of w after a from the rank arrays (a is in in(w) or out(w))
+            if d1:
+                base = int(g["outNext"][a]) if d2 else int(g["inSelf"][a]) + 1
+            else:
+                base = int(g["outSelf"][a]) + 1 if d2 else int(g["inNext"][a])
+            assert base == upper_bound(L2P, L2R[w], L2R[w + 1], a)
             hi2 = L2R[w + 1]
+            if base >= hi2:
+                continue
+            first2 = int(L2P[base])
+            A = lower_bound(pp, rb, s1_, min(a, end))
+            tlim = int(T[pp[A - 1]]) + delta
+            j = upper_bound(pp, s0, s1_, a)
             cur, valid, scan = base, 0, True
             while j < s1_:
                 pj = int(pp[j])
-                lo_j = max(rb, lower_bound(pp, s0, s1_, pend_of(g, int(T[pj]) - delta - 1)))
-                if lo_j >= A:
+                if int(T[pj]) > tlim:
                     break
+                if pj <= first2:
+                    j += 1
+                    continue
+                lo_j = max(rb, lower_bound(pp, s0, s1_, pend_of(g, int(T[pj]) - delta - 1)))
+                assert lo_j < A
                 if scan:
                     while cur < hi2 and cur - base < scan_max and L2P[cur] < pj:
                         y = int(L2N[cur])
