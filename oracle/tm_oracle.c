@@ -27,7 +27,9 @@
  * Pinned by: brute force from the match definition on random tiny graphs,
  * closed forms (parallel edges -> C(n,M), stars, cycles), and invariants
  * (root-range additivity, monotone in delta, relabel invariance) in
- * tests/test_oracle.py.
+ * tests/test_oracle.py; the work counters by their restatement over the
+ * enumerated prefix matches and the chosen lists' window counts in
+ * tests/test_oracle_work.py.
  */
 #include <pthread.h>
 #include <stdint.h>
