@@ -83,6 +83,9 @@ def main():
                          "--calibrate-copy into the per-launch HBM bytes of the roofline kernel (--json-out)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also copy this many bytes device-to-device (HBM counter scale for profiles)")
+    ap.add_argument("--constraints", action="store_true",
+                    help="instead: the constraint-feature ladder of the reference's additional-features "
+                         "queries (no / v / t / all / f) through the frontend on --graph")
     args = ap.parse_args()
     if args.reduce_pmc:
         reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1], args.json_out)
@@ -91,6 +94,9 @@ def main():
         print("bench.py measures one GPU: this build runs on a single MI355X "
               "(--gpus %d requested; the multi-GPU path is design-only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.constraints:
+        constraint_ladder(args)
+        return
 
     import numpy as np
     from everest_amd import abi, core
@@ -378,6 +384,76 @@ def serve(dg, motifs, counts, rate, args, n):
         res["served_qps"], res["latency_ms_p50"], res["latency_ms_p99"], res["plain_qps"],
         res["plain_latency_ms_p50"], res["plain_latency_ms_p99"], res["launches"]))
     return res
+
+
+# The reference's additional-features ladder (system/queries/additional-features/WI-M4-{no,v,t,all,f}
+# .yaml): the 4-cycle, then node-label predicates, temporal gaps, an anti-edge, a filter.  The
+# filter names the matched non-anti edges (the reference's names the anti edge's slot, e2).
+NODES4 = ["node0: f > 500000000", "node1: f < 500000000", "node2: f > 500000000", "node3: f < 500000000"]
+LADDER = [
+    ("no", "M4.txt", []),
+    ("v", "M4.txt", NODES4),
+    ("t", "M4.txt", NODES4 + ["temporal: [21600, 43200, 21600]"]),
+    ("all", "M4-A.txt", NODES4 + ["antiEdge: [False, False, True, False, False]",
+                                  "temporal: [21600, 43200, 43200, 21600]"]),
+    ("f", "M4-A.txt", NODES4 + ["antiEdge: [False, False, True, False, False]",
+                                "temporal: [21600, 43200, 43200, 21600]",
+                                'filter: "(e0 + e1 + e3 + e4) > 1000000000"']),
+]
+
+
+def constraint_ladder(args):
+    """Each query of LADDER at delta 86400 as the reference's frontend runs
+    it (system/tmine/frontend.py:52-96: YAML -> KernelGen -> plugin ->
+    GPUWorkerLibManager -> SingleGPUExecution; the constrained kernel for
+    v / t / all / f), on the synthetic graph loaded from arrays with its
+    seeded labels; the ladder runs twice and the second pass is reported (the
+    first uploads the graph)."""
+    import tempfile
+    from everest_amd import core
+    from everest_amd.gen import graphs
+    from everest_amd.tmine import KernelGen, Query, utils
+    root = tempfile.mkdtemp(prefix="everest_cons_")
+    t0 = time.time()
+    U, V, T = graphs.make(args.graph)
+    el = core.data.EdgeListLoader.fromArrays(args.graph, U, V, T)
+    hg = core.data.GraphDataLoader(el, core.data.FeatureLoader(el, "")).createGraphData()
+    log("graph %s: E=%d prepared in %.1fs" % (args.graph, hg.numEdges, time.time() - t0))
+    dm = core.data.MakeGraphDataManager(1)
+    lm = core.GPUWorkerLibManager()
+    lib = os.path.join(root, "libDYN.so")
+    res = []
+    for rep in range(2):
+        for name, pat, lines in LADDER:
+            q = os.path.join(root, "%s.yaml" % name)
+            with open(q, "w") as f:
+                f.write("pattern: %s\ngraph: %s.txt\nconstraints:\n  delta: 86400\n%s\noptions:\n  gpu: 1\n" % (
+                    pat, args.graph, "".join("  %s\n" % x for x in lines)))
+            query = Query.Query(q)
+            src = os.path.join(root, "%s.hip" % name)
+            with open(src, "w") as f:
+                f.write(KernelGen.KernelGen(query.pattern(), query).generate())
+            utils.compilePlugin(src, lib)
+            lm.openLib(lib)
+            w = lm.construct("DYN", 0)
+            ex = core.SingleGPUExecution(dm)
+            ex.setHostGraph(hg)
+            ex.setHostMotif(core.data.HostMotifData(query.pattern()))
+            ex.setWorker(w)
+            t = time.perf_counter()
+            ex.run(86400)
+            ms = (time.perf_counter() - t) * 1e3
+            cn = w.count()
+            del ex, w
+            log("pass %d %-3s %s: %d matches, %.1f ms" % (rep, name, pat, cn, ms))
+            if rep == 1:
+                res.append(dict(query=name, pattern=pat, constraints=lines, delta=86400, count=int(cn), ms=round(ms, 2)))
+    line = json.dumps({"constraint_ladder": res, "graph": args.graph,
+                       "source": "system/queries/additional-features/WI-M4-*.yaml shapes"})
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
 
 
 def canon(rows):

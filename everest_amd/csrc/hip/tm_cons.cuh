@@ -11,8 +11,9 @@
 // applies: every candidate of every motif edge is visited, in the
 // reference's order (system/templates/common/nextMatchInTask.snp:1-21: the
 // list of the edge's mapped endpoint after the previous edge's position,
-// until the first candidate past the window).  One thread per root, a
-// depth-first search over at most 8 levels; the per-query spec CS (generated
+// until the first candidate past the window).  A wavefront takes 64 roots
+// and deals their edge-1 candidates over its lanes; below edge 1 each lane
+// searches depth-first (at most 8 levels).  The per-query spec CS (generated
 // per query, as the reference generates its worker) supplies the motif
 // tables and the constraint functions:
 //   kM, kN                     motif edges and nodes
@@ -63,6 +64,119 @@ __device__ __forceinline__ void emit(const tm_query_dev &q, const tm_graph_view 
   if ((long long)w + 1 >= q.cap) __hip_atomic_store(q.stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Candidate list of motif edge j after position p: slots [*lo, *hi) of its
+// list (the pair's edges in m[a]'s out-list when both ends are mapped).
+template <class CS, int N>
+__device__ __forceinline__ void cand_list(const tm_graph_view &g, int j, const int (&m)[N], int p, int *lo, int *hi) {
+  const int k = CS::kind(j);
+  if (k == TM_K_NEW_DST) {
+    const int x = m[CS::a(j)];
+    *lo = tmk::upper_bound_i(g.outPos, g.outR[x], g.outR[x + 1], p);
+    *hi = g.outR[x + 1];
+  } else if (k == TM_K_NEW_SRC) {
+    const int y = m[CS::b(j)];
+    *lo = tmk::upper_bound_i(g.inPos, g.inR[y], g.inR[y + 1], p);
+    *hi = g.inR[y + 1];
+  } else {
+    const int x = m[CS::a(j)], y = m[CS::b(j)];
+    const int lo0 = g.outR[x], hi0 = g.outR[x + 1];
+    const int s0 = tmk::lower_bound_i(g.outPairNbr, lo0, hi0, y);
+    const int s1 = tmk::upper_bound_i(g.outPairNbr, s0, hi0, y);
+    *lo = tmk::upper_bound_i(g.outPairPos, s0, s1, p);
+    *hi = s1;
+  }
+}
+
+// Position and new node of slot s of edge j's list (node -1 when both ends are mapped).
+template <class CS>
+__device__ __forceinline__ int cand_pos(const tm_graph_view &g, int j, int s, int *node) {
+  const int k = CS::kind(j);
+  if (k == TM_K_NEW_DST) { *node = g.outNbr[s]; return g.outPos[s]; }
+  if (k == TM_K_NEW_SRC) { *node = g.inNbr[s]; return g.inPos[s]; }
+  *node = -1;
+  return g.outPairPos[s];
+}
+
+// Whether new node `node` of edge j passes: not mapped before, its label predicate.
+template <class CS, int N>
+__device__ __forceinline__ bool fresh_ok(const int *VF, int j, const int (&m)[N], int node) {
+  const int nm = CS::mapped(j);
+  bool fresh = true;
+  for (int i = 0; i < N; i++)
+    if (i < nm && m[i] == node) fresh = false;
+  return fresh && CS::node_ok(nm, VF, node);
+}
+
+// The search below a matched edge 1 (levels 2 .. M-1; M > 2), the root's
+// edges in m / pos / t0: every complete match emitted.
+template <class CS>
+__device__ void search_from2(const tm_graph_view &g, const int *EF, const int *VF, const tm_query_dev &q,
+                             bool enumerate, int (&m)[CS::kN], int (&pos)[CS::kM], long long t0, long long tmax,
+                             unsigned long long &cnt) {
+  constexpr int M = CS::kM;
+  int cur[M], lim[M];
+  long long tl[M];
+  int j = 2;
+  bool open = true;  // level j's candidate list is to be set up
+  while (j >= 2) {
+    if (open) {
+      open = false;
+      const int pj = CS::prev(j);
+      tl[j] = CS::window(j - 1, pj == 0 ? t0 : (long long)g.T[pos[pj]], t0, tmax);
+      cand_list<CS>(g, j, m, pos[pj], &cur[j], &lim[j]);
+    }
+    if (CS::anti(j)) {  // an anti edge: any candidate within the window rejects the partial match
+      bool hit = false;
+      while (!hit && cur[j] < lim[j]) {
+        const int p = g.outPairPos[cur[j]++];
+        if ((long long)g.T[p] > tl[j]) break;
+        hit = CS::edge_ok(j, EF, p);
+      }
+      const int pj = CS::prev(j);
+      if (!hit) {
+        pos[j] = pos[pj];
+        if (j < M - 1) {
+          j++;
+          open = true;
+          continue;
+        }
+        if (CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
+      }
+      j = pj;
+      continue;
+    }
+    if (cur[j] >= lim[j]) {
+      j = CS::prev(j);
+      continue;
+    }
+    int node;
+    const int p = cand_pos<CS>(g, j, cur[j]++, &node);
+    if ((long long)g.T[p] > tl[j]) {  // lists ascend in position, hence in time: the level is done
+      cur[j] = lim[j];
+      continue;
+    }
+    if (node >= 0) {
+      if (!fresh_ok<CS>(VF, j, m, node)) continue;
+      m[CS::mapped(j)] = node;
+    }
+    if (!CS::edge_ok(j, EF, p)) continue;
+    pos[j] = p;
+    if (j == M - 1) {
+      if (CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
+      continue;
+    }
+    j++;
+    open = true;
+  }
+}
+
+// One wavefront per 64 consecutive roots.  Each lane sets up its root and
+// the slots of edge 1's list inside the root's window; the wavefront deals
+// the union of those slots over its 64 lanes (a scan of the counts, each
+// slot's root found by a 6-step search over the lanes' sums), so a hub
+// root's thousands of edge-1 candidates are searched by 64 lanes; below
+// edge 1 each lane searches depth-first alone.  An anti edge 1 (a 2-edge
+// motif) is decided per root.
 template <class CS>
 __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
   const tm_graph_view &g = *args.gv;
@@ -74,94 +188,84 @@ __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
   constexpr int M = CS::kM, N = CS::kN;
   const tm_query_dev &q = args.queries[0];
   const bool enumerate = q.rows != nullptr;
+  const int lane = threadIdx.x;
   unsigned long long cnt = 0;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long rr = beg + (long long)blockIdx.x * blockDim.x + threadIdx.x; rr < end; rr += stride) {
-    const int r = (int)rr;
+  for (long long base = beg + (long long)blockIdx.x * 64; base < end; base += (long long)gridDim.x * 64) {
     if (enumerate && __hip_atomic_load(q.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-    const tm_edge e0 = g.Eg[r];
-    if (e0.u == e0.v) continue;  // roots with u == v are skipped (processFirstEdge.snp:1)
-    if (!CS::edge_ok(0, EF, r) || !CS::node_ok(0, VF, e0.u) || !CS::node_ok(1, VF, e0.v)) continue;
-    int m[N], pos[M], cur[M], lim[M];
-    long long tl[M];
-    m[0] = e0.u;
-    m[1] = e0.v;
-    pos[0] = r;
-    const long long t0 = e0.t, tmax = t0 + delta;
-    int j = 1;
-    bool open = true;  // level j's candidate list is to be set up
-    while (j >= 1) {
-      if (open) {
-        open = false;
-        const int pj = CS::prev(j);
-        const int p = pos[pj];
-        tl[j] = CS::window(j - 1, pj == 0 ? t0 : (long long)g.T[p], t0, tmax);
-        const int k = CS::kind(j);
-        if (k == TM_K_NEW_DST) {
-          const int x = m[CS::a(j)];
-          cur[j] = tmk::upper_bound_i(g.outPos, g.outR[x], g.outR[x + 1], p);
-          lim[j] = g.outR[x + 1];
-        } else if (k == TM_K_NEW_SRC) {
-          const int y = m[CS::b(j)];
-          cur[j] = tmk::upper_bound_i(g.inPos, g.inR[y], g.inR[y + 1], p);
-          lim[j] = g.inR[y + 1];
-        } else {  // both endpoints mapped: the pair's edges in x's out-list
-          const int x = m[CS::a(j)], y = m[CS::b(j)];
-          const int lo = g.outR[x], hi = g.outR[x + 1];
-          const int s0 = tmk::lower_bound_i(g.outPairNbr, lo, hi, y);
-          const int s1 = tmk::upper_bound_i(g.outPairNbr, s0, hi, y);
-          cur[j] = tmk::upper_bound_i(g.outPairPos, s0, s1, p);
-          lim[j] = s1;
-        }
-      }
-      if (CS::anti(j)) {  // an anti edge: any candidate within the window rejects the partial match
-        bool hit = false;
-        while (!hit && cur[j] < lim[j]) {
-          const int p = g.outPairPos[cur[j]++];
-          if ((long long)g.T[p] > tl[j]) break;
-          hit = CS::edge_ok(j, EF, p);
-        }
-        const int pj = CS::prev(j);
-        if (!hit) {
-          pos[j] = pos[pj];
-          if (j < M - 1) {
-            j++;
-            open = true;
-            continue;
+    const long long rr = base + lane;
+    int lo1 = 0, n1 = 0, u = 0, v = 0;
+    if (rr < end) {
+      const int r = (int)rr;
+      const tm_edge e0 = g.Eg[r];
+      u = e0.u;
+      v = e0.v;
+      // roots with u == v are skipped (processFirstEdge.snp:1)
+      if (u != v && CS::edge_ok(0, EF, r) && CS::node_ok(0, VF, u) && CS::node_ok(1, VF, v)) {
+        int m[N];
+        m[0] = u;
+        m[1] = v;
+        const long long t0 = e0.t;
+        const long long tl1 = CS::window(0, t0, t0, t0 + delta);
+        int hi1;
+        cand_list<CS>(g, 1, m, r, &lo1, &hi1);
+        if (CS::anti(1)) {  // 2-edge motif: the root alone, when no candidate of edge 1 is in the window
+          bool hit = false;
+          for (int s = lo1; !hit && s < hi1; s++) {
+            const int p = g.outPairPos[s];
+            if ((long long)g.T[p] > tl1) break;
+            hit = CS::edge_ok(1, EF, p);
           }
-          if (CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
+          int pos[M];
+          pos[0] = r;
+          pos[M - 1] = r;
+          if (!hit && CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
+        } else {
+          // the slots of edge 1 inside the window: positions before the first with a later time
+          const int pend = tmk::pend_of(g, tl1);
+          const int *P1 = CS::kind(1) == TM_K_NEW_DST ? g.outPos : (CS::kind(1) == TM_K_NEW_SRC ? g.inPos : g.outPairPos);
+          n1 = tmk::lower_bound_i(P1, lo1, hi1, pend) - lo1;
         }
-        j = pj;
-        continue;
       }
-      if (cur[j] >= lim[j]) {
-        j = CS::prev(j);
-        continue;
+    }
+    long long incl = n1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    const long long total = __shfl(incl, 63, 64);
+    for (long long b = 0; b < total; b += 64) {
+      const long long item = b + lane;
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (__shfl(incl, mid, 64) > item) hi = mid;
+        else lo = mid + 1;
       }
-      const int s = cur[j]++;
-      const int k = CS::kind(j);
-      const int p = k == TM_K_NEW_DST ? g.outPos[s] : (k == TM_K_NEW_SRC ? g.inPos[s] : g.outPairPos[s]);
-      if ((long long)g.T[p] > tl[j]) {  // lists ascend in position, hence in time: the level is done
-        cur[j] = lim[j];
-        continue;
+      const long long oincl = __shfl(incl, lo, 64);
+      const int on = __shfl(n1, lo, 64), olo1 = __shfl(lo1, lo, 64);
+      const int ou = __shfl(u, lo, 64), ov = __shfl(v, lo, 64);
+      if (item >= total) continue;
+      const int r = (int)(base + lo);
+      int m[N], pos[M];
+      m[0] = ou;
+      m[1] = ov;
+      pos[0] = r;
+      int node;
+      const int p = cand_pos<CS>(g, 1, olo1 + (int)(item - (oincl - on)), &node);
+      if (node >= 0) {
+        if (!fresh_ok<CS>(VF, 1, m, node)) continue;
+        m[CS::mapped(1)] = node;
       }
-      if (k != TM_K_CLOSE) {
-        const int node = k == TM_K_NEW_DST ? g.outNbr[s] : g.inNbr[s];
-        const int nm = CS::mapped(j);
-        bool fresh = true;
-        for (int i = 0; i < N; i++)
-          if (i < nm && m[i] == node) fresh = false;
-        if (!fresh || !CS::node_ok(nm, VF, node)) continue;
-        m[nm] = node;
-      }
-      if (!CS::edge_ok(j, EF, p)) continue;
-      pos[j] = p;
-      if (j == M - 1) {
+      if (!CS::edge_ok(1, EF, p)) continue;
+      pos[1] = p;
+      const long long t0 = g.T[r];
+      if constexpr (M == 2) {
+        (void)t0;
         if (CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
-        continue;
+      } else {
+        search_from2<CS>(g, EF, VF, q, enumerate, m, pos, t0, t0 + delta, cnt);
       }
-      j++;
-      open = true;
     }
   }
   // wavefront sum, one atomic per wavefront; this launch's counter rows zeroed

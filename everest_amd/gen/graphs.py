@@ -11,6 +11,10 @@ pl8m      1M nodes, 8M edges; Chung-Lu power law, Zipf(2.1) out/in weights
           bursty per-source Poisson times (rate LogNormal(0, 1.5)/day) over
           6 years.
 pl64m     2.6M nodes, 64M edges; same generator over 7.5 years.
+zipf8m    1M nodes, 8M edges; endpoints Zipf(1.0) over ranks (hubs of about
+          550k edges), times uniform over one year: dense in short temporal
+          cycles, like the reference's social-network inputs, for the
+          constraint-feature queries.
 """
 import hashlib
 
@@ -23,6 +27,7 @@ CONFIGS = {
     "zipf100k": dict(kind="zipf", N=10_000, E=100_000, alpha=2.0, span=30 * DAY, seed=1),
     "pl8m": dict(kind="pl", N=1_000_000, E=8_000_000, alpha=2.1, sigma=1.5, span=6 * YEAR, seed=2),
     "pl64m": dict(kind="pl", N=2_600_000, E=64_000_000, alpha=2.1, sigma=1.5, span=int(7.5 * YEAR), seed=3),
+    "zipf8m": dict(kind="zipf", N=1_000_000, E=8_000_000, alpha=1.0, span=YEAR, seed=7),
 }
 
 

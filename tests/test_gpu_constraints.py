@@ -53,6 +53,9 @@ CASES = [
      {}, {}, [3000, 3000, 2500, 2500, 3000], None, [False, False, False, True, True, False]),
     ("M4-A.txt", 4000, ["antiEdge: [false, false, true, false, false]", 'filter: "(e0 + e4) % 2 == 0"'], {}, {},
      None, lambda el, nl: (el[0] + el[4]) % 2 == 0, [False, False, True, False, False]),
+    # 2-edge motifs: a reply under a node predicate, and roots with no reply in the window
+    ("recip2.txt", 3000, ['node: "f % 2 == 0"'], {i: (lambda f: f % 2 == 0) for i in range(8)}, {}, None, None),
+    ("recip2.txt", 1500, ["antiEdge: [false, true]"], {}, {}, None, None, [False, True]),
 ]
 
 
@@ -102,7 +105,7 @@ def test_constraint_queries_through_the_frontend(world):
         say("  count", cn, "want", want)
         assert cn == want, (i, pat, lines, cn, want)
         nonzero += want > 0
-    assert nonzero >= 8
+    assert nonzero >= 10
 
 
 # enumerating constraint queries: (motif, delta, YAML lines, node preds, edge preds, temporal, filter)
