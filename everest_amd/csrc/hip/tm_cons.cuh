@@ -107,18 +107,18 @@ __device__ __forceinline__ bool fresh_ok(const int *VF, int j, const int (&m)[N]
   return fresh && CS::node_ok(nm, VF, node);
 }
 
-// The search below a matched edge 1 (levels 2 .. M-1; M > 2), the root's
-// edges in m / pos / t0: every complete match emitted.
+// The search below matched edges 0 .. j0-1 (levels j0 .. M-1; j0 < M), the
+// match so far in m / pos, the root's time t0: every complete match emitted.
 template <class CS>
-__device__ void search_from2(const tm_graph_view &g, const int *EF, const int *VF, const tm_query_dev &q,
-                             bool enumerate, int (&m)[CS::kN], int (&pos)[CS::kM], long long t0, long long tmax,
-                             unsigned long long &cnt) {
+__device__ void search_from(int j0, const tm_graph_view &g, const int *EF, const int *VF, const tm_query_dev &q,
+                            bool enumerate, int (&m)[CS::kN], int (&pos)[CS::kM], long long t0, long long tmax,
+                            unsigned long long &cnt) {
   constexpr int M = CS::kM;
   int cur[M], lim[M];
   long long tl[M];
-  int j = 2;
+  int j = j0;
   bool open = true;  // level j's candidate list is to be set up
-  while (j >= 2) {
+  while (j >= j0) {
     if (open) {
       open = false;
       const int pj = CS::prev(j);
@@ -174,9 +174,11 @@ __device__ void search_from2(const tm_graph_view &g, const int *EF, const int *V
 // the slots of edge 1's list inside the root's window; the wavefront deals
 // the union of those slots over its 64 lanes (a scan of the counts, each
 // slot's root found by a 6-step search over the lanes' sums), so a hub
-// root's thousands of edge-1 candidates are searched by 64 lanes; below
-// edge 1 each lane searches depth-first alone.  An anti edge 1 (a 2-edge
-// motif) is decided per root.
+// root's thousands of edge-1 candidates are searched by 64 lanes; each
+// batch of 64 edge-1 items deals its edge-2 slots the same way, and below
+// edge 2 each lane searches depth-first alone.  An anti edge 1 (a 2-edge
+// motif) is decided per root; an anti edge 2 per edge-1 item, and then edge
+// 3 is the dealt one.
 template <class CS>
 __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
   const tm_graph_view &g = *args.gv;
@@ -245,26 +247,106 @@ __global__ __launch_bounds__(64) void cons_kernel(tm_kernel_args args) {
       const long long oincl = __shfl(incl, lo, 64);
       const int on = __shfl(n1, lo, 64), olo1 = __shfl(lo1, lo, 64);
       const int ou = __shfl(u, lo, 64), ov = __shfl(v, lo, 64);
-      if (item >= total) continue;
-      const int r = (int)(base + lo);
+      // this lane's edge-1 item: the root's two nodes, edge 1's slot
       int m[N], pos[M];
+#pragma unroll
+      for (int i = 0; i < N; i++) m[i] = 0;
+#pragma unroll
+      for (int i = 0; i < M; i++) pos[i] = 0;
+      bool ok = item < total;
+      const int r = (int)(base + lo);
       m[0] = ou;
       m[1] = ov;
       pos[0] = r;
-      int node;
-      const int p = cand_pos<CS>(g, 1, olo1 + (int)(item - (oincl - on)), &node);
-      if (node >= 0) {
-        if (!fresh_ok<CS>(VF, 1, m, node)) continue;
-        m[CS::mapped(1)] = node;
+      if (ok) {
+        int node;
+        const int p = cand_pos<CS>(g, 1, olo1 + (int)(item - (oincl - on)), &node);
+        if (node >= 0) {
+          ok = fresh_ok<CS>(VF, 1, m, node);
+          if (ok) m[CS::mapped(1)] = node;
+        }
+        ok = ok && CS::edge_ok(1, EF, p);
+        pos[1] = p;
       }
-      if (!CS::edge_ok(1, EF, p)) continue;
-      pos[1] = p;
-      const long long t0 = g.T[r];
+      const long long t0 = ok ? (long long)g.T[r] : 0;
       if constexpr (M == 2) {
-        (void)t0;
-        if (CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
+        if (ok && CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
       } else {
-        search_from2<CS>(g, EF, VF, q, enumerate, m, pos, t0, t0 + delta, cnt);
+        // the dealt level: edge 2, or edge 3 after an anti edge 2 (checked per item here)
+        int jd = 2;
+        if (CS::anti(2)) {
+          if (ok) {
+            const long long tla = CS::window(1, g.T[pos[1]], t0, t0 + delta);
+            int la, ha;
+            cand_list<CS>(g, 2, m, pos[1], &la, &ha);
+            bool hit = false;
+            for (int s2 = la; !hit && s2 < ha; s2++) {
+              const int pa = g.outPairPos[s2];
+              if ((long long)g.T[pa] > tla) break;
+              hit = CS::edge_ok(2, EF, pa);
+            }
+            ok = !hit;
+            pos[2] = pos[1];
+          }
+          if (M == 3) {
+            if (ok && CS::filter(EF, VF, pos, m)) emit(q, g, pos, enumerate, cnt);
+            continue;
+          }
+          jd = 3;
+          if (CS::anti(3)) {  // two anti edges in a row: the rest per lane
+            if (ok) search_from<CS>(3, g, EF, VF, q, enumerate, m, pos, t0, t0 + delta, cnt);
+            continue;
+          }
+        }
+        // level jd dealt like edge 1: this lane's slots of it inside its window
+        const int pd = CS::prev(jd);
+        int lo2 = 0, n2 = 0;
+        if (ok) {
+          const long long tl2 = CS::window(jd - 1, g.T[pos[pd]], t0, t0 + delta);
+          int hi2;
+          cand_list<CS>(g, jd, m, pos[pd], &lo2, &hi2);
+          const int kd = CS::kind(jd);
+          const int *P2 = kd == TM_K_NEW_DST ? g.outPos : (kd == TM_K_NEW_SRC ? g.inPos : g.outPairPos);
+          n2 = tmk::lower_bound_i(P2, lo2, hi2, tmk::pend_of(g, tl2)) - lo2;
+        }
+        long long incl2 = n2;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const long long t = __shfl_up(incl2, o, 64);
+          if (lane >= o) incl2 += t;
+        }
+        const long long total2 = __shfl(incl2, 63, 64);
+        for (long long b2 = 0; b2 < total2; b2 += 64) {
+          const long long item2 = b2 + lane;
+          int l2 = 0, h2 = 63;
+          while (l2 < h2) {
+            const int mid = (l2 + h2) >> 1;
+            if (__shfl(incl2, mid, 64) > item2) h2 = mid;
+            else l2 = mid + 1;
+          }
+          const long long oincl2 = __shfl(incl2, l2, 64);
+          const int on2 = __shfl(n2, l2, 64), olo2 = __shfl(lo2, l2, 64);
+          int mm[N], pp[M];
+#pragma unroll
+          for (int i = 0; i < N; i++) mm[i] = __shfl(m[i], l2, 64);
+#pragma unroll
+          for (int i = 0; i < M; i++) pp[i] = __shfl(pos[i], l2, 64);
+          const long long ot0 = __shfl(t0, l2, 64);
+          if (item2 >= total2) continue;
+          int node;
+          const int p = cand_pos<CS>(g, jd, olo2 + (int)(item2 - (oincl2 - on2)), &node);
+          if (node >= 0) {
+            if (!fresh_ok<CS>(VF, jd, mm, node)) continue;
+            mm[CS::mapped(jd)] = node;
+          }
+          if (!CS::edge_ok(jd, EF, p)) continue;
+          pp[jd] = p;
+          if (jd == M - 1) {
+            if (CS::filter(EF, VF, pp, mm)) emit(q, g, pp, enumerate, cnt);
+          } else {
+            search_from<CS>(jd + 1, g, EF, VF, q, enumerate, mm, pp, ot0, ot0 + delta, cnt);
+          }
+        }
       }
     }
   }
