@@ -547,7 +547,7 @@ def cpu_baseline(hg, arrs, motifs, dg, budget_s):
 
     g = dict(E=hg.numEdges, Eg=arrs["Eg"].reshape(-1, 3), inV=arrs["inEdgesV"], outV=arrs["outEdgesV"],
              inR=arrs["inEdgesR"], outR=arrs["outEdgesR"], Euid=arrs["Euid"])
-    # the host cores this process may use (a GPU box grants 16 per GPU)
+    # the host cores this process may use (16 per GPU on the measurement hosts)
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
     E = hg.numEdges
     nwin, cal = 4, 4
