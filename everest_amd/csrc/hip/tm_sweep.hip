@@ -83,7 +83,9 @@ __global__ __launch_bounds__(256) void sweep_kernel(tm_graph_view g, const tm_qu
       if (q < end && u != v) {  // q as a root
         const int x = it.anchor == 0 ? u : v;
         const int pend = pend_of(g, (long long)e.t + delta);
-        const int lo = same ? LS[q] + 1 : upper_bound_i(LP, LR[x], LR[x + 1], q);
+        // the window's first slot: after q's own slot, or (the other list of q's
+        // own endpoint) the upload's rank arrays
+        const int lo = same ? LS[q] + 1 : (it.anchor == 0 ? g.inNext[q] : g.outNext[q]);
         const int hi = lower_bound_i(LP, lo, LR[x + 1], pend);
         const long long nu = pcnt(g, out, x, u, q, pend), nv = pcnt(g, out, x, v, q, pend);
         const unsigned long long cc = c2(nu) + c2(nv);
@@ -91,8 +93,8 @@ __global__ __launch_bounds__(256) void sweep_kernel(tm_graph_view g, const tm_qu
       }
       if (q > beg) {  // q as a slot of the anchor list of x
         const int x = out ? u : v, w = out ? v : u;
-        const int r0 = lower_bound_i(RP, RR[x], RR[x + 1], beg);
-        const int r1 = lower_bound_i(RP, r0, RR[x + 1], end);
+        const int r0 = beg == 0 ? RR[x] : lower_bound_i(RP, RR[x], RR[x + 1], beg);
+        const int r1 = (RR[x + 1] > r0 && RP[RR[x + 1] - 1] < end) ? RR[x + 1] : lower_bound_i(RP, r0, RR[x + 1], end);
         if (r0 == r1) continue;
         // first root whose window end reaches q: time >= t_q - delta, i.e.
         // position >= the first position with that time
@@ -219,7 +221,8 @@ __global__ __launch_bounds__(256) void pair_sweep_kernel(tm_graph_view g, const 
       long long valid = 0;
       bool scan = true;
       int loj = -1;
-      for (int j = upper_bound_i(pp, os0, os1, a); j < os1; j++) {
+      // the first pair edge after a: A itself when a < end (a is no pair edge and a > r >= beg)
+      for (int j = a < end ? A : upper_bound_i(pp, os0, os1, a); j < os1; j++) {
         const int pj = pp[j];
         const long long tpj = g.T[pj];
         if (tpj > tlim) break;

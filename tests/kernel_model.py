@@ -1346,16 +1346,19 @@ def window_sweep_count(g, md, delta, beg, end):
             pend = pend_of(g, T[q] + delta)
             if same:
                 lo = (g["outSelf"] if out else g["inSelf"])[q] + 1
-            else:
-                lo = upper_bound(LP, LR[x], LR[x + 1], q)
+            else:  # the other list of q's own endpoint: the rank arrays
+                lo = int((g["inNext"] if anc == 0 else g["outNext"])[q])
+                assert lo == upper_bound(LP, LR[x], LR[x + 1], q)
             hi = lower_bound(LP, lo, LR[x + 1], pend)
             nu, nv = pcnt(x, u, q, pend), pcnt(x, v, q, pend)
             c2 = _c2(nu) + _c2(nv)
             tot += _c2(hi - lo - nu - nv) + c2 if pat == WS_STAR else -c2
         if q > beg:  # slot of q in the list of x
             x, w = (u, v) if out else (v, u)
-            r0 = lower_bound(RP, RR[x], RR[x + 1], beg)
-            r1 = lower_bound(RP, r0, RR[x + 1], end)
+            r0 = RR[x] if beg == 0 else lower_bound(RP, RR[x], RR[x + 1], beg)
+            r1 = RR[x + 1] if (RR[x + 1] > r0 and RP[RR[x + 1] - 1] < end) else lower_bound(RP, r0, RR[x + 1], end)
+            assert (r0, r1) == (lower_bound(RP, RR[x], RR[x + 1], beg),
+                                lower_bound(RP, lower_bound(RP, RR[x], RR[x + 1], beg), RR[x + 1], end))
             if r0 == r1:
                 continue
             # first root whose window end reaches q: time >= T[q] - delta
@@ -1470,7 +1473,8 @@ def pair_sweep_count(g, md, delta, beg, end, scan_max=32):
             first2 = int(L2P[base])
             A = lower_bound(pp, rb, s1_, min(a, end))
             tlim = int(T[pp[A - 1]]) + delta
-            j = upper_bound(pp, s0, s1_, a)
+            j = A if a < end else upper_bound(pp, s0, s1_, a)  # a is not a pair edge, and a > r >= beg
+            assert j == upper_bound(pp, s0, s1_, a)
             cur, valid, scan = base, 0, True
             while j < s1_:
                 pj = int(pp[j])
