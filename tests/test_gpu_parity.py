@@ -423,6 +423,10 @@ def test_repeated_pair_rewrites():
         dg.close()
 
 
+PAIR_SHAPES = [[(0, 1), (s1, 2) if d1 else (2, s1), (2, 3) if d2 else (3, 2), (0, 1)]
+               for s1 in (0, 1) for d1 in (1, 0) for d2 in (0, 1)]
+
+
 SWEEP_SHAPES = [[(0, 1), (0, 2), (0, 3)], [(0, 1), (2, 1), (3, 1)], [(0, 1), (1, 2), (1, 3)],
                 [(0, 1), (2, 0), (3, 0)], [(0, 1), (0, 2), (0, 2)], [(0, 1), (2, 1), (2, 1)],
                 [(0, 1), (1, 2), (1, 2)], [(0, 1), (2, 0), (2, 0)]]
@@ -476,7 +480,8 @@ def test_window_sweep():
 
 
 def test_window_sweep_server():
-    """Swept queries through the server, alone and mixed with mined ones."""
+    """Swept queries (window sweep and root-pair sweep) through the server,
+    mixed with mined ones, equal the oracle."""
     abi = _abi()
     rng = np.random.default_rng(29)
     U, V, T = random_graph(rng, 3000, 60, 1500)
@@ -484,6 +489,7 @@ def test_window_sweep_server():
     dg = abi.DeviceGraph.from_dict(g)
     qs = [([(a, b, i) for i, (a, b) in enumerate(e)], int(rng.integers(0, 200))) for e in SWEEP_SHAPES]
     qs += [([(0, 1, 0), (1, 2, 1), (2, 0, 2)], 150), ([(0, 1, 0), (0, 2, 1), (2, 3, 2), (3, 0, 3)], 150)]
+    qs += [([(a, b, i) for i, (a, b) in enumerate(e)], int(rng.integers(0, 200))) for e in PAIR_SHAPES]
     want = [oracle.count(g, m, d, nthreads=4) for m, d in qs]
     say("window sweep server: %d queries" % len(qs))
     srv = abi.Server(dg, max_batch=16, tune=abi.tuning(sweep=1))
@@ -524,10 +530,6 @@ def test_window_ends_at_last_root_pair_edge():
         assert c == want
         assert dg.batch_count([(mot, delta), (mot, delta)]) == [want, want]
         dg.close()
-
-
-PAIR_SHAPES = [[(0, 1), (s1, 2) if d1 else (2, s1), (2, 3) if d2 else (3, 2), (0, 1)]
-               for s1 in (0, 1) for d1 in (1, 0) for d2 in (0, 1)]
 
 
 def test_pair_sweep():
